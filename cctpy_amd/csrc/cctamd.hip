@@ -1,0 +1,608 @@
+// cctamd: Biot-Savart field and RK4 tracking of protons through discretised
+// CCT windings, written for MI355X (gfx950, CDNA4, wave64).
+//
+// Kernel map (one HIP stream, no allocation inside the library):
+//   k_winding_generate  CCT parameters -> FP64 segments in the global frame
+//   k_pack_fp32         FP64 segments  -> FP32 tiles re-centred on a per-tile
+//                                         FP64 origin
+//   k_bs_partial<P,N>   split-K Biot-Savart: one workgroup = one point block x
+//                       one chunk of tiles; the 4 waves share the block and
+//                       interleave tiles; segments are wave-uniform (scalar
+//                       loads into SGPRs), points live in VGPRs (N per lane)
+//   k_field_reduce      fixed-order sum of the chunk partials + QS + uniform
+//   k_rk4_stage<S>      same reduction, then RK4 stage S of each particle
+//
+// The C ABI is declared in include/cctamd.h.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/cctamd.h"
+
+namespace {
+
+constexpr int kTile = CCTAMD_TILE;
+constexpr int kWave = 64;
+constexpr int kWavesPerWG = 4;
+constexpr int kWG = kWave * kWavesPerWG;
+
+// ------------------------------------------------------------ geometry
+// Mirrors CCT.__init__/phi_ksi_function/BipolarToroidalCoordinateSystem.convert
+// (final_code/packages/cct.py:96-164, 166-188, 213-224) with the same
+// operation order; contraction is off so each product is rounded like Python's.
+#pragma clang fp contract(off)
+__device__ inline void cct_point(const double* __restrict__ prm, int64_t i, double out[3]) {
+    const double ksi = prm[15] + prm[16] * (double)i;
+    double phi = prm[17] * ksi + prm[18];
+    const int ntilt = (int)prm[20];
+    const int skip = (int)prm[29];
+    for (int j = 0; j < ntilt && j < CCTAMD_MAX_TILTS; ++j) {
+        if ((skip >> j) & 1) continue;
+        phi = phi + prm[21 + j] * sin((double)(j + 1) * ksi);
+    }
+    const double t = prm[12] / (prm[14] - cos(ksi));
+    out[0] = t * prm[13] * cos(phi);
+    out[1] = t * prm[13] * sin(phi);
+    out[2] = t * sin(ksi);
+}
+
+__global__ void k_winding_generate(const double* __restrict__ params, const int64_t* __restrict__ count_off,
+                                   const int64_t* __restrict__ dst, int64_t ncct, int64_t nseg_total,
+                                   double* __restrict__ seg64) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nseg_total) return;
+    int64_t lo = 0, hi = ncct - 1;  // last c with count_off[c] <= g
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (count_off[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const double* prm = params + lo * CCTAMD_CCT_PARAMS;
+    const int64_t i = g - count_off[lo];
+    double p0[3], p1[3];
+    cct_point(prm, i, p0);
+    cct_point(prm, i + 1, p1);
+    const double coef = prm[19];
+    const double kl[3] = {coef * (p1[0] - p0[0]), coef * (p1[1] - p0[1]), coef * (p1[2] - p0[2])};
+    const double m[3] = {0.5 * (p1[0] + p0[0]), 0.5 * (p1[1] + p0[1]), 0.5 * (p1[2] + p0[2])};
+    const double* O = prm;
+    const double* XI = prm + 3;
+    const double* YI = prm + 6;
+    const double* ZI = prm + 9;
+    double* out = seg64 + (dst[lo] + i) * 8;
+    for (int c = 0; c < 3; ++c) {
+        out[c] = XI[c] * kl[0] + YI[c] * kl[1] + ZI[c] * kl[2];
+        out[4 + c] = O[c] + (XI[c] * m[0] + YI[c] * m[1] + ZI[c] * m[2]);
+    }
+    out[3] = 0.0;
+    out[7] = 0.0;
+}
+__global__ void k_winding_points(const double* __restrict__ params, int64_t npoints, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoints) return;
+    double p[3];
+    cct_point(params, i, p);
+    out[3 * i] = p[0];
+    out[3 * i + 1] = p[1];
+    out[3 * i + 2] = p[2];
+}
+#pragma clang fp contract(fast)
+
+__global__ void k_pack_fp32(const double* __restrict__ seg64, int64_t nseg, float* __restrict__ seg32,
+                            double* __restrict__ tile_org) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg) return;
+    const int64_t t = i / kTile;
+    const double* s = seg64 + i * 8;
+    const double* o = seg64 + t * kTile * 8 + 4;  // tile origin: first midpoint of the tile
+    float* d = seg32 + i * 8;
+    d[0] = (float)s[0];
+    d[1] = (float)s[1];
+    d[2] = (float)s[2];
+    d[3] = 0.0f;
+    d[4] = (float)(s[4] - o[0]);
+    d[5] = (float)(s[5] - o[1]);
+    d[6] = (float)(s[6] - o[2]);
+    d[7] = 0.0f;
+    if (i % kTile == 0) {
+        tile_org[t * 4 + 0] = o[0];
+        tile_org[t * 4 + 1] = o[1];
+        tile_org[t * 4 + 2] = o[2];
+        tile_org[t * 4 + 3] = 0.0;
+    }
+}
+
+// ----------------------------------------------------- Biot-Savart core
+struct Work {
+    int32_t design, pt_start, pt_count, tile_start, tile_count, r0, r1, r2;
+};
+
+template <int PPL>
+__device__ inline void load_points(const double* __restrict__ pts, int64_t ptot, int pt_start, int pt_count,
+                                   int lane, double px[PPL], double py[PPL], double pz[PPL]) {
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+        int idx = lane + kWave * j;
+        idx = idx < pt_count ? idx : 0;  // surplus lanes duplicate point 0, results discarded
+        const int64_t g = (int64_t)pt_start + idx;
+        px[j] = pts[g];
+        py[j] = pts[ptot + g];
+        pz[j] = pts[2 * ptot + g];
+    }
+}
+
+// The 4 waves of the workgroup hold partial sums for the same point block;
+// they are added in wave order (deterministic) and written as one partial.
+template <int PPL>
+__device__ inline void write_partial(double ax[PPL], double ay[PPL], double az[PPL], int wave, int lane,
+                                     double* __restrict__ out) {
+    constexpr int PB = kWave * PPL;
+    __shared__ double red[kWavesPerWG][3][PB];
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+        red[wave][0][lane + kWave * j] = ax[j];
+        red[wave][1][lane + kWave * j] = ay[j];
+        red[wave][2][lane + kWave * j] = az[j];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 3 * PB; k += kWG) {
+        const int c = k / PB, i = k % PB;
+        out[k] = ((red[0][c][i] + red[1][c][i]) + red[2][c][i]) + red[3][c][i];
+    }
+}
+
+// FP32 terms on tiles re-centred at an FP64 origin; each tile's 32 terms are
+// summed in FP32 and the tile sums are accumulated in FP64.
+template <int PPL>
+__global__ __launch_bounds__(kWG) void k_bs_partial_f32(const float* __restrict__ seg32,
+                                                         const double* __restrict__ tile_org,
+                                                         const Work* __restrict__ work,
+                                                         const double* __restrict__ pts, int64_t ptot,
+                                                         double* __restrict__ partial) {
+    constexpr int PB = kWave * PPL;
+    const Work w = work[blockIdx.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double px[PPL], py[PPL], pz[PPL];
+    load_points<PPL>(pts, ptot, w.pt_start, w.pt_count, lane, px, py, pz);
+    double ax[PPL], ay[PPL], az[PPL];
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) ax[j] = ay[j] = az[j] = 0.0;
+
+    const int tend = w.tile_start + w.tile_count;
+    for (int t = w.tile_start + wave; t < tend; t += kWavesPerWG) {
+        const double ox = tile_org[4 * t], oy = tile_org[4 * t + 1], oz = tile_org[4 * t + 2];
+        float qx[PPL], qy[PPL], qz[PPL], sx[PPL], sy[PPL], sz[PPL];
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            qx[j] = (float)(px[j] - ox);
+            qy[j] = (float)(py[j] - oy);
+            qz[j] = (float)(pz[j] - oz);
+            sx[j] = sy[j] = sz[j] = 0.0f;
+        }
+        const float4* s = reinterpret_cast<const float4*>(seg32) + (int64_t)t * kTile * 2;
+#pragma unroll 8
+        for (int k = 0; k < kTile; ++k) {
+            const float4 kl = s[2 * k];
+            const float4 m = s[2 * k + 1];
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                const float rx = qx[j] - m.x, ry = qy[j] - m.y, rz = qz[j] - m.z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float inv3 = inv * inv * inv;
+                sx[j] += (kl.y * rz - kl.z * ry) * inv3;
+                sy[j] += (kl.z * rx - kl.x * rz) * inv3;
+                sz[j] += (kl.x * ry - kl.y * rx) * inv3;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            ax[j] += (double)sx[j];
+            ay[j] += (double)sy[j];
+            az[j] += (double)sz[j];
+        }
+    }
+    write_partial<PPL>(ax, ay, az, wave, lane, partial + (int64_t)blockIdx.x * 3 * PB);
+}
+
+// Plain FP64 terms on global-frame segments.
+template <int PPL>
+__global__ __launch_bounds__(kWG) void k_bs_partial_f64(const double* __restrict__ seg64,
+                                                         const Work* __restrict__ work,
+                                                         const double* __restrict__ pts, int64_t ptot,
+                                                         double* __restrict__ partial) {
+    constexpr int PB = kWave * PPL;
+    const Work w = work[blockIdx.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double px[PPL], py[PPL], pz[PPL];
+    load_points<PPL>(pts, ptot, w.pt_start, w.pt_count, lane, px, py, pz);
+    double ax[PPL], ay[PPL], az[PPL];
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) ax[j] = ay[j] = az[j] = 0.0;
+
+    const int tend = w.tile_start + w.tile_count;
+    for (int t = w.tile_start + wave; t < tend; t += kWavesPerWG) {
+        const double2* s = reinterpret_cast<const double2*>(seg64) + (int64_t)t * kTile * 4;
+#pragma unroll 4
+        for (int k = 0; k < kTile; ++k) {
+            const double2 k01 = s[4 * k], k2_ = s[4 * k + 1], m01 = s[4 * k + 2], m2_ = s[4 * k + 3];
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                const double rx = px[j] - m01.x, ry = py[j] - m01.y, rz = pz[j] - m2_.x;
+                const double r2 = rx * rx + ry * ry + rz * rz;
+                const double inv = rsqrt(r2);
+                const double inv3 = inv * inv * inv;
+                ax[j] += (k01.y * rz - k2_.x * ry) * inv3;
+                ay[j] += (k2_.x * rx - k01.x * rz) * inv3;
+                az[j] += (k01.x * ry - k01.y * rx) * inv3;
+            }
+        }
+    }
+    write_partial<PPL>(ax, ay, az, wave, lane, partial + (int64_t)blockIdx.x * 3 * PB);
+}
+
+// ------------------------------------------------ reduction + QS + RK4
+#pragma clang fp contract(off)
+// Hard-edge QS field (final_code/packages/magnets.py:637-668), same comparisons.
+__device__ inline void qs_add(const double* __restrict__ q, const double p[3], double b[3]) {
+    const double d0 = p[0] + (-q[0]), d1 = p[1] + (-q[1]), d2 = p[2] + (-q[2]);
+    const double x = q[3] * d0 + q[4] * d1 + q[5] * d2;
+    const double y = q[6] * d0 + q[7] * d1 + q[8] * d2;
+    const double z = q[9] * d0 + q[10] * d1 + q[11] * d2;
+    const double L = q[12], g = q[13], s = q[14], a = q[15];
+    if (z < 0 || z > L) return;
+    if (fabs(x) > a || fabs(y) > a || sqrt(x * x + y * y) > a) return;
+    const double bx = g * y + s * (x * y);
+    const double by = g * x + 0.5 * s * (x * x - y * y);
+    b[0] = b[0] + (q[3] * bx + q[6] * by);
+    b[1] = b[1] + (q[4] * bx + q[7] * by);
+    b[2] = b[2] + (q[5] * bx + q[8] * by);
+}
+#pragma clang fp contract(fast)
+
+__device__ inline int64_t find_design(const int64_t* __restrict__ off, int64_t nd, int64_t g) {
+    int64_t lo = 0, hi = nd - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Sum of the chunk partials of point g (one wave per point, fixed lane order,
+// butterfly reduction: deterministic), then QS and uniform contributions.
+__device__ inline bool point_field(int64_t g, int ppl, const double* __restrict__ partial,
+                                   const int64_t* __restrict__ design_wbase, const int32_t* __restrict__ design_nchunk,
+                                   const int64_t* __restrict__ design_pt_off, int64_t nd,
+                                   const double* __restrict__ qs, const int64_t* __restrict__ design_qs_off,
+                                   const double* __restrict__ bconst, const double p[3], double b[3]) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t d = find_design(design_pt_off, nd, g);
+    const int pb = kWave * ppl;
+    const int64_t lp = g - design_pt_off[d];
+    const int64_t blk = lp / pb;
+    const int i = (int)(lp % pb);
+    const int nc = design_nchunk[d];
+    const int64_t base = design_wbase[d] + blk * nc;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int c = lane; c < nc; c += kWave) {
+        const double* pr = partial + (base + c) * 3 * pb;
+        s0 += pr[i];
+        s1 += pr[pb + i];
+        s2 += pr[2 * pb + i];
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off);
+        s1 += __shfl_xor(s1, off);
+        s2 += __shfl_xor(s2, off);
+    }
+    if (lane != 0) return false;
+    b[0] = s0;
+    b[1] = s1;
+    b[2] = s2;
+    for (int64_t k = design_qs_off[d]; k < design_qs_off[d + 1]; ++k) qs_add(qs + 16 * k, p, b);
+    b[0] = b[0] + bconst[4 * d];
+    b[1] = b[1] + bconst[4 * d + 1];
+    b[2] = b[2] + bconst[4 * d + 2];
+    return true;
+}
+
+__global__ __launch_bounds__(kWG) void k_field_reduce(int ppl, const double* __restrict__ partial,
+                                                      const int64_t* __restrict__ design_wbase,
+                                                      const int32_t* __restrict__ design_nchunk,
+                                                      const int64_t* __restrict__ design_pt_off, int64_t nd,
+                                                      const double* __restrict__ qs,
+                                                      const int64_t* __restrict__ design_qs_off,
+                                                      const double* __restrict__ bconst,
+                                                      const double* __restrict__ pts, int64_t ptot,
+                                                      double* __restrict__ B) {
+    const int64_t g = (int64_t)blockIdx.x * kWavesPerWG + (threadIdx.x >> 6);
+    if (g >= ptot) return;
+    const double p[3] = {pts[g], pts[ptot + g], pts[2 * ptot + g]};
+    double b[3];
+    if (point_field(g, ppl, partial, design_wbase, design_nchunk, design_pt_off, nd, qs, design_qs_off, bconst, p, b)) {
+        B[g] = b[0];
+        B[ptot + g] = b[1];
+        B[2 * ptot + g] = b[2];
+    }
+}
+
+// scratch rows: 0-2 stage position (read by k_bs_partial), 3-5 stage
+// velocity, 6-8 running sum of position slopes, 9-11 of velocity slopes.
+__global__ void k_rk4_init(const double* __restrict__ state, int64_t ptot, double* __restrict__ scratch,
+                           double* __restrict__ record) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ptot) return;
+    for (int c = 0; c < 6; ++c) {
+        scratch[c * ptot + g] = state[c * ptot + g];
+        if (record) record[c * ptot + g] = state[c * ptot + g];
+    }
+}
+
+#pragma clang fp contract(off)
+// RK4 stage S of final_code/packages/base_utils.py:303-309 with the callback
+// of particles.py:526-535: a = (e/m) * (v x B), k = [v, a]; Y1 = Y0 + dt/2 k1,
+// Y2 = Y0 + dt/2 k2, Y3 = Y0 + dt k3, Y0 += dt/6 (((k1 + 2 k2) + 2 k3) + k4).
+template <int S>
+__global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __restrict__ partial,
+                                                   const int64_t* __restrict__ design_wbase,
+                                                   const int32_t* __restrict__ design_nchunk,
+                                                   const int64_t* __restrict__ design_pt_off, int64_t nd,
+                                                   const double* __restrict__ qs,
+                                                   const int64_t* __restrict__ design_qs_off,
+                                                   const double* __restrict__ bconst, double* __restrict__ state,
+                                                   double* __restrict__ scratch, int64_t ptot,
+                                                   const double* __restrict__ kq, const double* __restrict__ dt,
+                                                   const int64_t* __restrict__ nsteps, int64_t step,
+                                                   double* __restrict__ record) {
+    const int64_t g = (int64_t)blockIdx.x * kWavesPerWG + (threadIdx.x >> 6);
+    if (g >= ptot) return;
+    if (step >= nsteps[g]) return;  // finished particles keep their state
+    double* ps = scratch;
+    double* vs = scratch + 3 * ptot;
+    double* kp = scratch + 6 * ptot;
+    double* kv = scratch + 9 * ptot;
+    const double p[3] = {ps[g], ps[ptot + g], ps[2 * ptot + g]};
+    double b[3];
+    if (!point_field(g, ppl, partial, design_wbase, design_nchunk, design_pt_off, nd, qs, design_qs_off, bconst, p, b))
+        return;
+    const double v[3] = {vs[g], vs[ptot + g], vs[2 * ptot + g]};
+    const double k = kq[g];
+    const double cr[3] = {v[1] * b[2] - v[2] * b[1], -v[0] * b[2] + v[2] * b[0], v[0] * b[1] - v[1] * b[0]};
+    const double a[3] = {cr[0] * k, cr[1] * k, cr[2] * k};
+    const double h = dt[g];
+    const double h2 = h / 2;
+    for (int c = 0; c < 3; ++c) {
+        const int64_t o = c * ptot + g;
+        const double p0 = state[o], v0 = state[3 * ptot + o];
+        if (S == 0) {
+            kp[o] = v[c];
+            kv[o] = a[c];
+            ps[o] = p0 + v[c] * h2;
+            vs[o] = v0 + a[c] * h2;
+        } else if (S == 1) {
+            kp[o] = kp[o] + v[c] * 2;
+            kv[o] = kv[o] + a[c] * 2;
+            ps[o] = p0 + v[c] * h2;
+            vs[o] = v0 + a[c] * h2;
+        } else if (S == 2) {
+            kp[o] = kp[o] + v[c] * 2;
+            kv[o] = kv[o] + a[c] * 2;
+            ps[o] = p0 + v[c] * h;
+            vs[o] = v0 + a[c] * h;
+        } else {
+            const double h6 = h / 6;
+            const double np = p0 + (kp[o] + v[c]) * h6;
+            const double nv = v0 + (kv[o] + a[c]) * h6;
+            state[o] = np;
+            state[3 * ptot + o] = nv;
+            ps[o] = np;
+            vs[o] = nv;
+            if (record) {
+                double* r = record + (step + 1) * 6 * ptot;
+                r[o] = np;
+                r[3 * ptot + o] = nv;
+            }
+        }
+    }
+}
+#pragma clang fp contract(fast)
+
+// ------------------------------------------------------------ launchers
+inline int hip_status(hipError_t e) { return e == hipSuccess ? CCTAMD_OK : (int)e; }
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+template <int PPL>
+hipError_t launch_bs(int precision, const void* seg, const double* tile_org, const Work* work, int64_t nwork,
+                     const double* pts, int64_t ptot, double* partial, hipStream_t st) {
+    if (nwork == 0) return hipSuccess;
+    if (precision == 32)
+        hipLaunchKernelGGL(k_bs_partial_f32<PPL>, dim3((unsigned)nwork), dim3(kWG), 0, st,
+                           (const float*)seg, tile_org, work, pts, ptot, partial);
+    else
+        hipLaunchKernelGGL(k_bs_partial_f64<PPL>, dim3((unsigned)nwork), dim3(kWG), 0, st,
+                           (const double*)seg, work, pts, ptot, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_bs_any(int precision, int ppl, const void* seg, const double* tile_org, const Work* work,
+                         int64_t nwork, const double* pts, int64_t ptot, double* partial, hipStream_t st) {
+    switch (ppl) {
+        case 1: return launch_bs<1>(precision, seg, tile_org, work, nwork, pts, ptot, partial, st);
+        case 2: return launch_bs<2>(precision, seg, tile_org, work, nwork, pts, ptot, partial, st);
+        default: return launch_bs<4>(precision, seg, tile_org, work, nwork, pts, ptot, partial, st);
+    }
+}
+
+bool valid_common(int precision, int ppl, const void* seg, const double* tile_org, int64_t nwork,
+                  int64_t ndesign) {
+    if (precision != 32 && precision != 64) return false;
+    if (ppl != 1 && ppl != 2 && ppl != 4) return false;
+    if (nwork < 0 || ndesign <= 0 || nwork > 0x7fffffff) return false;
+    if (nwork > 0 && seg == nullptr) return false;
+    if (nwork > 0 && precision == 32 && tile_org == nullptr) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cctamd_version(void) { return 100; }
+
+const char* cctamd_error_string(int code) {
+    if (code == CCTAMD_OK) return "ok";
+    if (code == CCTAMD_EINVAL) return "cctamd: invalid argument";
+    if (code == CCTAMD_ERANGE) return "cctamd: size out of range";
+    if (code > 0) return hipGetErrorString((hipError_t)code);
+    return "cctamd: unknown error";
+}
+
+int cctamd_winding_generate(const double* params, const int64_t* count_off, const int64_t* dst, int64_t ncct,
+                            int64_t nseg_total, double* seg64, void* stream) {
+    if (ncct < 0 || nseg_total < 0) return CCTAMD_EINVAL;
+    if (ncct == 0 || nseg_total == 0) return CCTAMD_OK;
+    if (!params || !count_off || !dst || !seg64) return CCTAMD_EINVAL;
+    const int threads = 256;
+    const int64_t blocks = ceil_div(nseg_total, threads);
+    if (blocks > 0x7fffffff) return CCTAMD_ERANGE;
+    hipLaunchKernelGGL(k_winding_generate, dim3((unsigned)blocks), dim3(threads), 0, (hipStream_t)stream, params,
+                       count_off, dst, ncct, nseg_total, seg64);
+    return hip_status(hipGetLastError());
+}
+
+int cctamd_winding_points(const double* params, int64_t npoints, double* out, void* stream) {
+    if (npoints < 0) return CCTAMD_EINVAL;
+    if (npoints == 0) return CCTAMD_OK;
+    if (!params || !out) return CCTAMD_EINVAL;
+    const int64_t blocks = ceil_div(npoints, 256);
+    if (blocks > 0x7fffffff) return CCTAMD_ERANGE;
+    hipLaunchKernelGGL(k_winding_points, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, params, npoints,
+                       out);
+    return hip_status(hipGetLastError());
+}
+
+int cctamd_pack_fp32(const double* seg64, int64_t nseg, float* seg32, double* tile_org, void* stream) {
+    if (nseg < 0 || nseg % kTile) return CCTAMD_EINVAL;
+    if (nseg == 0) return CCTAMD_OK;
+    if (!seg64 || !seg32 || !tile_org) return CCTAMD_EINVAL;
+    const int threads = 256;
+    const int64_t blocks = ceil_div(nseg, threads);
+    if (blocks > 0x7fffffff) return CCTAMD_ERANGE;
+    hipLaunchKernelGGL(k_pack_fp32, dim3((unsigned)blocks), dim3(threads), 0, (hipStream_t)stream, seg64, nseg,
+                       seg32, tile_org);
+    return hip_status(hipGetLastError());
+}
+
+int64_t cctamd_plan(int64_t ndesign, const int64_t* design_ntile, const int64_t* design_npt,
+                    const int64_t* design_tile_off, const int64_t* design_pt_off, int ppl, int64_t target_items,
+                    int64_t min_tiles, int32_t* work, int64_t* design_wbase, int32_t* design_nchunk) {
+    if (ndesign <= 0 || !design_ntile || !design_npt) return CCTAMD_EINVAL;
+    if (ppl != 1 && ppl != 2 && ppl != 4) return CCTAMD_EINVAL;
+    if (target_items <= 0 || min_tiles <= 0) return CCTAMD_EINVAL;
+    const int64_t pb = kWave * ppl;
+    int64_t total = 0;
+    for (int64_t d = 0; d < ndesign; ++d) {
+        if (design_ntile[d] < 0 || design_npt[d] < 0) return CCTAMD_EINVAL;
+        total += ceil_div(design_npt[d], pb) * design_ntile[d];
+    }
+    const int64_t per_item = std::max(min_tiles, ceil_div(std::max<int64_t>(total, 1), target_items));
+    int64_t n = 0;
+    for (int64_t d = 0; d < ndesign; ++d) {
+        const int64_t nb = ceil_div(design_npt[d], pb);
+        const int64_t T = design_ntile[d];
+        const int64_t nc = (nb > 0 && T > 0) ? ceil_div(T, per_item) : 0;
+        if (work) {
+            if (!design_tile_off || !design_pt_off || !design_wbase || !design_nchunk) return CCTAMD_EINVAL;
+            if (nc > 0x7fffffff) return CCTAMD_ERANGE;
+            design_wbase[d] = n;
+            design_nchunk[d] = (int32_t)nc;
+            for (int64_t b = 0; b < nb && nc > 0; ++b) {
+                for (int64_t c = 0; c < nc; ++c) {
+                    const int64_t t0 = T * c / nc, t1 = T * (c + 1) / nc;
+                    const int64_t ps = design_pt_off[d] + b * pb;
+                    const int64_t pc = std::min(pb, design_npt[d] - b * pb);
+                    const int64_t ts = design_tile_off[d] + t0;
+                    if (ps + pc > 0x7fffffff || ts + (t1 - t0) > 0x7fffffff) return CCTAMD_ERANGE;
+                    int32_t* w = work + CCTAMD_WORK_INTS * (n + b * nc + c);
+                    w[0] = (int32_t)d;
+                    w[1] = (int32_t)ps;
+                    w[2] = (int32_t)pc;
+                    w[3] = (int32_t)ts;
+                    w[4] = (int32_t)(t1 - t0);
+                    w[5] = w[6] = w[7] = 0;
+                }
+            }
+        }
+        n += nb * nc;
+    }
+    return n;
+}
+
+int cctamd_bs_field(int precision, int ppl, const void* seg, const double* tile_org, const int32_t* work,
+                    int64_t nwork, const int64_t* design_wbase, const int32_t* design_nchunk,
+                    const int64_t* design_pt_off, int64_t ndesign, const double* qs, const int64_t* design_qs_off,
+                    const double* bconst, const double* points, int64_t npoints, double* partial, double* B,
+                    void* stream) {
+    if (!valid_common(precision, ppl, seg, tile_org, nwork, ndesign)) return CCTAMD_EINVAL;
+    if (npoints < 0) return CCTAMD_EINVAL;
+    if (npoints == 0) return CCTAMD_OK;
+    if (!points || !B || !design_wbase || !design_nchunk || !design_pt_off || !design_qs_off || !bconst)
+        return CCTAMD_EINVAL;
+    if (nwork > 0 && (!work || !partial)) return CCTAMD_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = launch_bs_any(precision, ppl, seg, tile_org, (const Work*)work, nwork, points, npoints, partial, st);
+    if (e != hipSuccess) return (int)e;
+    const int64_t blocks = ceil_div(npoints, kWavesPerWG);
+    if (blocks > 0x7fffffff) return CCTAMD_ERANGE;
+    hipLaunchKernelGGL(k_field_reduce, dim3((unsigned)blocks), dim3(kWG), 0, st, ppl, partial, design_wbase,
+                       design_nchunk, design_pt_off, ndesign, qs, design_qs_off, bconst, points, npoints, B);
+    return hip_status(hipGetLastError());
+}
+
+int cctamd_track_rk4(int precision, int ppl, const void* seg, const double* tile_org, const int32_t* work,
+                     int64_t nwork, const int64_t* design_wbase, const int32_t* design_nchunk,
+                     const int64_t* design_pt_off, int64_t ndesign, const double* qs, const int64_t* design_qs_off,
+                     const double* bconst, double* state, int64_t nparticles, const double* kq, const double* dt,
+                     const int64_t* nsteps, int64_t max_steps, double* scratch, double* partial, double* record,
+                     void* stream) {
+    if (!valid_common(precision, ppl, seg, tile_org, nwork, ndesign)) return CCTAMD_EINVAL;
+    if (nparticles < 0 || max_steps < 0) return CCTAMD_EINVAL;
+    if (nparticles == 0) return CCTAMD_OK;
+    if (!state || !kq || !dt || !nsteps || !scratch || !design_wbase || !design_nchunk || !design_pt_off ||
+        !design_qs_off || !bconst)
+        return CCTAMD_EINVAL;
+    if (nwork > 0 && (!work || !partial)) return CCTAMD_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t eblocks = ceil_div(nparticles, 256);
+    const int64_t rblocks = ceil_div(nparticles, kWavesPerWG);
+    if (rblocks > 0x7fffffff) return CCTAMD_ERANGE;
+    hipLaunchKernelGGL(k_rk4_init, dim3((unsigned)eblocks), dim3(256), 0, st, state, nparticles, scratch, record);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    const Work* wk = (const Work*)work;
+    for (int64_t step = 0; step < max_steps; ++step) {
+#define CCTAMD_STAGE(S)                                                                                          \
+    e = launch_bs_any(precision, ppl, seg, tile_org, wk, nwork, scratch, nparticles, partial, st);               \
+    if (e != hipSuccess) return (int)e;                                                                          \
+    hipLaunchKernelGGL(k_rk4_stage<S>, dim3((unsigned)rblocks), dim3(kWG), 0, st, ppl, partial, design_wbase,   \
+                       design_nchunk, design_pt_off, ndesign, qs, design_qs_off, bconst, state, scratch,         \
+                       nparticles, kq, dt, nsteps, step, record);                                                \
+    e = hipGetLastError();                                                                                       \
+    if (e != hipSuccess) return (int)e;
+        CCTAMD_STAGE(0)
+        CCTAMD_STAGE(1)
+        CCTAMD_STAGE(2)
+        CCTAMD_STAGE(3)
+#undef CCTAMD_STAGE
+    }
+    return CCTAMD_OK;
+}
+
+}  // extern "C"

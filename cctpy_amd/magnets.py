@@ -1,0 +1,215 @@
+"""Magnet interface and the analytic magnets of the tracking path
+(final_code/packages/magnets.py:49-67, 417-437, 539-759).
+
+Field values are computed on the GPU: a magnet (or a combination of magnets)
+is flattened into a device design (CCT windings, QS records, uniform field)
+and evaluated by libcctamd.  Magnet kinds the device does not know are still
+accepted by the field API, which adds their own ``magnetic_field_at`` on top;
+tracking through them is rejected (as the reference GPU path does,
+final_code/packages/gpu_accelerator.py:1970).
+"""
+from __future__ import annotations
+
+import math
+from typing import Callable, Iterable, List, Sequence, Tuple
+
+import numpy
+
+from .base_utils import BaseUtils
+from .constants import MM
+from .line2s import Line2
+from .local_coordinate_system import LocalCoordinateSystem
+from .point import P2, P3, ValueWithDistance
+
+
+class ApertureObject:
+    def is_out_of_aperture(self, point: P3) -> bool:
+        raise NotImplementedError
+
+
+class Magnet:
+    """Anything with a field: ``magnetic_field_at(point: P3) -> P3``."""
+
+    def magnetic_field_at(self, point: P3) -> P3:
+        return self.magnetic_field_at_points([point])[0]
+
+    def magnetic_field_at_points(self, points: Sequence[P3], precision: int = 64) -> List[P3]:
+        """Batched field evaluation (one GPU launch for all points)."""
+        from . import engine
+        arr = numpy.array([p.to_list() for p in points], dtype=numpy.float64).reshape(-1, 3)
+        b = field_of(self, arr, precision=precision)
+        return [P3(*row) for row in b]
+
+    def magnetic_field_along(self, line2: Line2, p2_t0_p3: Callable[[P2], P3] = lambda p2: P3(p2.x, p2.y, 0.0),
+                             step: float = 1 * MM) -> List[ValueWithDistance[P3]]:
+        """Field sampled at int(length/step)+1 points of line2 (magnets.py:69-91), one GPU launch."""
+        length = line2.get_length()
+        distances = BaseUtils.linspace(0, length, int(length / step) + 1)
+        pts = [line2.point_at(d).to_p3(transformation=p2_t0_p3) for d in distances]
+        return [ValueWithDistance(b, d) for b, d in zip(self.magnetic_field_at_points(pts), distances)]
+
+    def magnetic_field_bz_along(self, line2: Line2, p2_t0_p3: Callable[[P2], P3] = lambda p2: P3(p2.x, p2.y, 0.0),
+                                step: float = 1 * MM) -> List[P2]:
+        return [P2(v.distance, v.value.z) for v in self.magnetic_field_along(line2, p2_t0_p3=p2_t0_p3, step=step)]
+
+    @staticmethod
+    def no_magnet() -> "Magnet":
+        return UniformMagnet(P3.zeros())
+
+    @staticmethod
+    def uniform_magnet(magnetic_field: P3 = None) -> "Magnet":
+        return UniformMagnet(P3.zeros() if magnetic_field is None else magnetic_field)
+
+    @staticmethod
+    def combine(*magnets) -> "CombinedMagnet":
+        for m in magnets:
+            if not isinstance(m, Magnet):
+                raise ValueError(f"{m} is not a magnet")
+        return CombinedMagnet(*magnets)
+
+    # -- device description ------------------------------------------------
+    def _device_parts(self, out: "DeviceParts") -> None:
+        out.foreign.append(self)
+
+
+class DeviceParts:
+    """Flattened magnet list of one design, in append order."""
+
+    def __init__(self):
+        self.cct_params: List[numpy.ndarray] = []
+        self.cct_counts: List[int] = []
+        self.qs: List[numpy.ndarray] = []
+        self.bconst = numpy.zeros(3)
+        self.foreign: List[Magnet] = []
+
+    def spec(self):
+        from .engine import DesignSpec
+        from ._native import CCT_PARAMS
+        return DesignSpec(
+            cct_params=numpy.array(self.cct_params, numpy.float64).reshape(len(self.cct_params), CCT_PARAMS),
+            cct_counts=numpy.array(self.cct_counts, numpy.int64),
+            qs=numpy.array(self.qs, numpy.float64).reshape(len(self.qs), 16),
+            bconst=self.bconst.copy())
+
+
+def device_parts(magnet: Magnet) -> DeviceParts:
+    parts = DeviceParts()
+    magnet._device_parts(parts)
+    return parts
+
+
+def field_of(magnet: Magnet, points: numpy.ndarray, precision: int = 64) -> numpy.ndarray:
+    """B (n, 3) of one magnet at global points (n, 3)."""
+    from . import engine
+    parts = device_parts(magnet)
+    pts = numpy.asarray(points, numpy.float64).reshape(-1, 3)
+    model = engine.model_for([parts.spec()], fp32=(precision == 32))
+    b = model.field(pts, [pts.shape[0]], precision=precision)
+    for m in parts.foreign:
+        for i in range(pts.shape[0]):
+            f = m.magnetic_field_at(P3(*pts[i]))
+            b[i] += (f.x, f.y, f.z)
+    return b
+
+
+class UniformMagnet(Magnet):
+    """Same field everywhere (magnets.py:417-437)."""
+
+    def __init__(self, magnetic_field: P3 = None) -> None:
+        self.magnetic_field = P3.zeros() if magnetic_field is None else magnetic_field
+
+    def magnetic_field_at(self, point: P3) -> P3:
+        return self.magnetic_field.copy()
+
+    def _device_parts(self, out: DeviceParts) -> None:
+        out.bconst = out.bconst + numpy.array(self.magnetic_field.to_list())
+
+    def __str__(self) -> str:
+        return f"UniformMagnet, field {self.magnetic_field}"
+
+
+class CombinedMagnet(Magnet):
+    """Sum of several magnets (magnets.py:539-586)."""
+
+    def __init__(self, *magnets) -> None:
+        for m in magnets:
+            if not isinstance(m, Magnet):
+                raise ValueError(f"{m} is not a magnet")
+        self._magnets: List[Magnet] = list(magnets)
+
+    def add(self, magnet: Magnet) -> "CombinedMagnet":
+        self._magnets.append(magnet)
+        return self
+
+    def add_all(self, magnets: Iterable[Magnet]) -> "CombinedMagnet":
+        self._magnets.extend(magnets)
+        return self
+
+    def get_magnets(self) -> List[Magnet]:
+        return self._magnets
+
+    def remove(self, m: Magnet) -> "CombinedMagnet":
+        self._magnets.remove(m)
+        return self
+
+    def magnetic_field_at(self, point: P3) -> P3:
+        return Magnet.magnetic_field_at(self, point)
+
+    def _device_parts(self, out: DeviceParts) -> None:
+        for m in self._magnets:
+            m._device_parts(out)
+
+
+class QS(Magnet, ApertureObject):
+    """Hard-edge quadrupole + sextupole (magnets.py:589-759).
+
+    Local frame: origin at the entrance centre, z along the orbit; inside
+    0 <= z <= length and the aperture, Bx = g y + s x y, By = g x + s (x^2 - y^2)/2.
+    """
+
+    def __init__(self, local_coordinate_system: LocalCoordinateSystem, length: float, gradient: float,
+                 second_gradient: float, aperture_radius: float):
+        self.local_coordinate_system = local_coordinate_system
+        self.length = float(length)
+        self.gradient = float(gradient)
+        self.second_gradient = float(second_gradient)
+        self.aperture_radius = float(aperture_radius)
+
+    def __str__(self) -> str:
+        return (f"QS:local_coordinate_system={self.local_coordinate_system}, length={self.length}, "
+                f"gradient={self.gradient}, second_gradient={self.second_gradient}, "
+                f"aperture_radius={self.aperture_radius}")
+
+    __repr__ = __str__
+
+    def magnetic_field_at(self, point: P3) -> P3:
+        return Magnet.magnetic_field_at(self, point)
+
+    def is_out_of_aperture(self, point: P3) -> bool:
+        lp = self.local_coordinate_system.point_to_local_coordinate(point)
+        if 0 <= lp.z <= self.length:
+            return (lp.x ** 2 + lp.y ** 2) > self.aperture_radius ** 2
+        return False
+
+    @staticmethod
+    def create_qs_along(trajectory: Line2, s: float, length: float, gradient: float, second_gradient: float,
+                        aperture_radius: float) -> "QS":
+        """Entrance at trajectory(s), z along the orbit, x = z rotated +90 deg (magnets.py:687-724)."""
+        origin = trajectory.point_at(s)
+        z = trajectory.direct_at(s)
+        x = z.rotate(BaseUtils.angle_to_radian(90))
+        lcs = LocalCoordinateSystem(location=origin.to_p3(), x_direction=x.to_p3(), z_direction=z.to_p3())
+        return QS(lcs, length, gradient, second_gradient, aperture_radius)
+
+    def to_numpy_array(self, numpy_dtype=numpy.float64) -> numpy.ndarray:
+        """[O, XI, YI, ZI, length, gradient, second_gradient, aperture_radius] (magnets.py:726-748)."""
+        return numpy.array(self.local_coordinate_system.to_array()
+                           + [self.length, self.gradient, self.second_gradient, self.aperture_radius],
+                           dtype=numpy_dtype)
+
+    def _device_parts(self, out: DeviceParts) -> None:
+        out.qs.append(self.to_numpy_array())
+
+    @staticmethod
+    def as_qs(anything) -> "QS":
+        return anything

@@ -1,0 +1,127 @@
+/*
+ * cctamd — C ABI of the MI355X (gfx950) Biot-Savart + RK4 tracking library.
+ *
+ * Every device pointer below is caller-owned device memory (the library never
+ * allocates or frees it); every call is stream-ordered on `stream`
+ * (a hipStream_t passed as void*, NULL = default stream) and returns
+ * CCTAMD_OK, a negative CCTAMD_E* code for invalid arguments, or a positive
+ * hipError_t.  No function keeps state between calls.
+ *
+ * Data layout in HBM (one "segment set" holds the CCT windings of D designs):
+ *   seg64      double[S][8]  {klx, kly, klz, 0, mx, my, mz, 0}, global frame,
+ *              kl = 1e-7*I*(P[i+1]-P[i]), m = (P[i+1]+P[i])/2; S is a multiple of
+ *              CCTAMD_TILE; each design's range is padded with kl=0 segments.
+ *   seg32      float[S][8]   {klx, kly, klz, 0, m-c, 0} with c = tile_org[tile]
+ *   tile_org   double[S/CCTAMD_TILE][4]  per-tile FP64 origin (first midpoint)
+ *   qs         double[Q][16] {O, XI, YI, ZI, length, gradient, second_gradient,
+ *              aperture_radius} (the reference's QS record)
+ *   bconst     double[D][4]  uniform field per design (sum of uniform magnets)
+ *   points / particle state: SoA double[3][P] (x row, y row, z row)
+ *   work       int32[W][8]   work items from cctamd_plan()
+ */
+#ifndef CCTAMD_H
+#define CCTAMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCTAMD_OK 0
+#define CCTAMD_EINVAL (-22)
+#define CCTAMD_ERANGE (-34)
+
+#define CCTAMD_TILE 32          /* segments per tile */
+#define CCTAMD_CCT_PARAMS 32    /* doubles per CCT record for cctamd_winding_generate */
+#define CCTAMD_MAX_TILTS 8
+#define CCTAMD_WORK_INTS 8
+
+/* Library version (major*10000 + minor*100 + patch). */
+int cctamd_version(void);
+
+/* Message for a return code of this library. */
+const char *cctamd_error_string(int code);
+
+/*
+ * Winding discretisation on the device.
+ * Replaces CCT.__init__ path generation + global_current_elements_and_
+ * elementary_current_positions (final_code/packages/cct.py:36-164, 166-188,
+ * 213-224, 423-457), whose output the reference feeds to its GPU kernels
+ * (final_code/packages/gpu_accelerator.py:1960-1966).
+ *   params     double[ncct][CCTAMD_CCT_PARAMS] (device):
+ *              [0..11] frame O, XI, YI, ZI; [12] a; [13] sinh(eta); [14] cosh(eta);
+ *              [15] ksi0; [16] dksi = (ksi1-ksi0)*(1/(W*n)); [17] phi slope k;
+ *              [18] phi intercept b; [19] 1e-7*current; [20] number of tilt terms;
+ *              [21..28] tilt coefficients cot(t_j)/((j+1) sinh(eta)); [29] bit mask of
+ *              tilt terms skipped (tilt == 90 deg); [30..31] unused.
+ *   count_off  int64[ncct+1] (device): exclusive prefix sum of segment counts W*n
+ *   dst        int64[ncct]   (device): first seg64 row of each CCT
+ */
+int cctamd_winding_generate(const double *params, const int64_t *count_off, const int64_t *dst,
+                            int64_t ncct, int64_t nseg_total, double *seg64, void *stream);
+
+/*
+ * Path points of one CCT in its local frame (the reference's dispersed_path3,
+ * final_code/packages/cct.py:134-148): out double[npoints][3], point i at
+ * ksi = ksi0 + dksi*i, params as for cctamd_winding_generate (frame unused).
+ */
+int cctamd_winding_points(const double *params, int64_t npoints, double *out, void *stream);
+
+/* FP32 tile packing of a FP64 segment set (nseg a multiple of CCTAMD_TILE). */
+int cctamd_pack_fp32(const double *seg64, int64_t nseg, float *seg32, double *tile_org, void *stream);
+
+/*
+ * Host-only planner: splits (design, point block, tile chunk) work.
+ *   design_ntile[d], design_npt[d], design_tile_off[d], design_pt_off[d] (host).
+ *   ppl: points per lane (1, 2 or 4); a point block holds 64*ppl points.
+ * With work == NULL returns the number of work items only. Otherwise fills
+ * work[nwork][8] = {design, pt_start, pt_count, tile_start, tile_count, 0, 0, 0},
+ * design_wbase[d] (first item of design d) and design_nchunk[d].
+ * Returns the number of items, or a negative code.
+ */
+int64_t cctamd_plan(int64_t ndesign, const int64_t *design_ntile, const int64_t *design_npt,
+                    const int64_t *design_tile_off, const int64_t *design_pt_off, int ppl,
+                    int64_t target_items, int64_t min_tiles, int32_t *work, int64_t *design_wbase,
+                    int32_t *design_nchunk);
+
+/*
+ * Field of D designs at their points: B = sum over CCT segments + QS + uniform.
+ * Replaces GPU_ACCELERATOR.current_element_B / magnet_at_qs_dates /
+ * magnet_at_beamline_with_single_qs (final_code/packages/gpu_accelerator.py:1147-1196,
+ * 1287-1347, 1349-1431) and CCT/Beamline.magnetic_field_at (cct.py:249-285,
+ * beamline.py:46-53).
+ *   precision: 64 (seg64, FP64 sum) or 32 (seg32 + tile_org, FP32 terms,
+ *              FP64 accumulation across tiles).
+ *   partial: device workspace of nwork*3*64*ppl doubles.
+ */
+int cctamd_bs_field(int precision, int ppl, const void *seg, const double *tile_org,
+                    const int32_t *work, int64_t nwork, const int64_t *design_wbase,
+                    const int32_t *design_nchunk, const int64_t *design_pt_off, int64_t ndesign,
+                    const double *qs, const int64_t *design_qs_off, const double *bconst,
+                    const double *points, int64_t npoints, double *partial, double *B, void *stream);
+
+/*
+ * RK4 tracking of D designs' particles through their own fields.
+ * Replaces GPU_ACCELERATOR.track_multi_particle_beamline_for_magnet_with_multi_qs
+ * (final_code/packages/gpu_accelerator.py:1912-2044) and the CPU ParticleRunner.run_only
+ * (final_code/packages/particles.py:561-632 with base_utils.py:263-311).
+ *   state    double[6][P] in/out: x, y, z, vx, vy, vz rows.
+ *   scratch  double[12][P] device workspace.
+ *   kq[P] = e/m, dt[P], nsteps[P] (int64) per particle; steps beyond nsteps[i]
+ *   leave particle i unchanged; max_steps = max(nsteps).
+ *   record   nullable double[max_steps+1][6][P]: state after every step.
+ */
+int cctamd_track_rk4(int precision, int ppl, const void *seg, const double *tile_org,
+                     const int32_t *work, int64_t nwork, const int64_t *design_wbase,
+                     const int32_t *design_nchunk, const int64_t *design_pt_off, int64_t ndesign,
+                     const double *qs, const int64_t *design_qs_off, const double *bconst,
+                     double *state, int64_t nparticles, const double *kq, const double *dt,
+                     const int64_t *nsteps, int64_t max_steps, double *scratch, double *partial,
+                     double *record, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CCTAMD_H */

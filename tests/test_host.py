@@ -1,0 +1,137 @@
+"""CPU tests: C ABI exports, host planner, and host-side geometry against the oracle."""
+import ctypes
+import math
+import re
+
+import numpy as np
+import pytest
+
+from tests._helpers import ROOT, ensure_built, rel
+
+ensure_built()
+
+from cctpy_amd import _native as N  # noqa: E402
+from cctpy_amd.cctpy import (CCT, P2, P3, Beamline, HUST_SC_GANTRY, LocalCoordinateSystem,  # noqa: E402
+                             ParticleFactory, PhaseSpaceParticle, Trajectory)
+from oracle import cct_oracle as O  # noqa: E402
+from tests.golden.params import SECTION1_DEMO  # noqa: E402
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(f"{ROOT}/include/cctamd.h").read()
+    declared = set(re.findall(r"\b(cctamd_[a-z0-9_]+)\s*\(", header))
+    assert declared == set(N.EXPORTED)
+    for name in declared:
+        assert hasattr(N.lib, name)
+    assert N.lib.cctamd_version() >= 100
+    assert N.lib.cctamd_error_string(-22) == b"cctamd: invalid argument"
+
+
+def _plan(ntile, npt, ppl, target, min_tiles=4):
+    ntile = np.asarray(ntile, np.int64)
+    npt = np.asarray(npt, np.int64)
+    toff = np.concatenate([[0], np.cumsum(ntile)[:-1]]).astype(np.int64)
+    poff = np.concatenate([[0], np.cumsum(npt)]).astype(np.int64)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    n = N.lib.cctamd_plan(len(ntile), p(ntile), p(npt), p(toff), p(poff), ppl, target, min_tiles, None, None, None)
+    work = np.zeros((max(n, 1), 8), np.int32)
+    wbase = np.zeros(len(ntile), np.int64)
+    nch = np.zeros(len(ntile), np.int32)
+    n2 = N.lib.cctamd_plan(len(ntile), p(ntile), p(npt), p(toff), p(poff), ppl, target, min_tiles, p(work),
+                           p(wbase), p(nch))
+    assert n2 == n
+    return work[:n], wbase, nch, toff, poff
+
+
+@pytest.mark.parametrize("ppl", [1, 2, 4])
+def test_plan_covers_every_point_tile_pair_once(ppl):
+    ntile = [37, 0, 5, 1200, 3]
+    npt = [300, 17, 0, 64, 1]
+    work, wbase, nch, toff, poff = _plan(ntile, npt, ppl, target=200)
+    pb = 64 * ppl
+    cover = {}
+    for d in range(len(ntile)):
+        nb = -(-npt[d] // pb)
+        if ntile[d] == 0 or npt[d] == 0:
+            assert nch[d] == 0
+            continue
+        for b in range(nb):
+            for c in range(nch[d]):
+                w = work[wbase[d] + b * nch[d] + c]
+                assert w[0] == d and w[1] == poff[d] + b * pb and w[2] == min(pb, npt[d] - b * pb)
+                for t in range(w[3], w[3] + w[4]):
+                    for q in range(w[1], w[1] + w[2]):
+                        cover[(t, q)] = cover.get((t, q), 0) + 1
+    expect = {(toff[d] + t, poff[d] + q) for d in range(len(ntile)) for t in range(ntile[d]) for q in range(npt[d])}
+    assert set(cover) == expect and all(v == 1 for v in cover.values())
+
+
+def test_plan_rejects_bad_arguments():
+    z = np.zeros(1, np.int64)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    assert N.lib.cctamd_plan(1, p(z), p(z), p(z), p(z), 3, 10, 4, None, None, None) == -22
+    assert N.lib.cctamd_plan(0, p(z), p(z), p(z), p(z), 2, 10, 4, None, None, None) == -22
+    neg = np.array([-1], np.int64)
+    assert N.lib.cctamd_plan(1, p(neg), p(z), p(z), p(z), 2, 10, 4, None, None, None) == -22
+
+
+def _frame_eq(lcs: LocalCoordinateSystem, fr: O.Frame):
+    for a, b in ((lcs.location, fr.O), (lcs.XI, fr.XI), (lcs.YI, fr.YI), (lcs.ZI, fr.ZI)):
+        assert a.to_list() == list(b)
+
+
+def test_host_placement_matches_oracle_bitwise():
+    ours = HUST_SC_GANTRY(**SECTION1_DEMO).create_second_bending_part_beamline()
+    ref = O.hust_gantry_section("second", **SECTION1_DEMO)
+    assert ours.get_length() == ref.length
+    assert len(ours.magnets) == len(ref.magnets)
+    for a, b in zip(ours.magnets, ref.magnets):
+        _frame_eq(a.local_coordinate_system, b.frame)
+    for s in np.linspace(0, ref.length, 37):
+        assert ours.point_at(float(s)).to_list() == list(ref.traj.point_at(float(s)))
+        assert ours.direct_at(float(s)).to_list() == list(ref.traj.direct_at(float(s)))
+
+
+def test_cct_device_record_reproduces_phi():
+    traj = Trajectory.set_start_point(P2.origin()).first_line(P2.x_direct(), 1).add_arc_line(1, True, 30)
+    cct = CCT.create_cct_along(traj, 1, 1, 0.06, 30, [90, 30, 75.5], 30, 1000, P2(0, 0),
+                               P2(30 * 2 * math.pi, -30 / 180 * math.pi))
+    ref = O.create_cct_along(O.Trajectory((0, 0), (1, 0), 1).add_arc(1, True, 30), 1, 1, 0.06, 30, [90, 30, 75.5],
+                             30, 1000, (0, 0), (30 * 2 * math.pi, -30 / 180 * math.pi))
+    rec = cct.device_record()
+    ksi = rec[15] + rec[16] * np.arange(0, 3601, 97, dtype=np.float64)
+    phi = rec[17] * ksi + rec[18]
+    for j in range(int(rec[20])):
+        if not (int(rec[29]) >> j) & 1:
+            phi = phi + rec[21 + j] * np.sin((j + 1) * ksi)
+    assert np.max(np.abs(phi - ref.phi_of_ksi(ksi))) < 1e-15
+    assert int(rec[29]) == 1  # the 90 degree term is skipped
+    assert rec[19] == 1e-7 * 1000
+
+
+def test_host_particles_match_oracle(golden):
+    g = golden["factory"]
+    ip = ParticleFactory.create_proton(P3.origin(), P3.x_direct(), 215)
+    cs = ip.get_natural_coordinate_system(y_direction=P3.z_direct())
+    pp = PhaseSpaceParticle(x=3.5e-3, xp=7.5e-3, y=0, yp=0, z=0, delta=0.1)
+    p = ParticleFactory.create_from_phase_space_particle(ip, cs, pp)
+    assert p.position.to_list() == g["phase_space_particle"]["position"]
+    assert rel(p.velocity.to_list(), g["phase_space_particle"]["velocity"]) < 1e-15
+    back = PhaseSpaceParticle.create_from_running_particle(ip, cs, p)
+    assert np.allclose([back.x, back.xp, back.y, back.yp, back.z, back.delta], g["phase_space_back"], rtol=1e-14,
+                       atol=1e-18)
+
+
+def test_beamline_builder_errors():
+    bl = Beamline.set_start_point().first_drift(length=1)
+    with pytest.raises(ValueError):
+        bl.append_agcct(1, [0.1, 0.2, 0.3, 0.4], [10], [[30], [30]], [[10], [5]], [1, 1])
+    with pytest.raises(ValueError):
+        bl.append_agcct(1, [0.1, 0.2, 0.3], [10], [[30], [30]], [[10], [5]], [1, 1])
+    with pytest.raises(ValueError):
+        CCT(LocalCoordinateSystem(), 0.1, 0.2, 30, [30], 10, 1, P2(), P2(1, 1))
+
+
+def test_single_hip_runtime_in_process():
+    """The library shares torch's HIP runtime (one libamdhip64 mapped), so torch buffers are valid for it."""
+    assert len(N.loaded_runtimes()) == 1
