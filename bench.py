@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Throughput of RK4 proton tracking through a CCT gantry section on one MI355X.
+
+A *step* is one full tracking pass of the whole synthetic beam through the
+workload's beamline (all RK4 steps; default workload ``agcct_section``:
+368,640 winding segments + 2 quads, 512 protons, 2,510 RK4 steps of 1 mm).
+The timed passes use the FP32 Biot-Savart path (per-tile re-centred FP32
+terms, FP64 accumulation, FP64 particle state); one pass of the plain FP64
+path is timed beside it and is the reference for the final-position error.
+Inputs are resident in HBM before timing starts; every pass restarts from the
+same initial state with a device-to-device copy.
+
+Prints one JSON line on stdout (diagnostics go to stderr).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+FLOP_PER_PAIR = 27           # r (3) + |r|^2 (5) + sqrt (1) + r^3 (2) + kl x r (9) + 1/x (1) + scale (3) + sum (3)
+PEAK_TFLOPS = {32: 157.3, 64: 78.6}
+METRIC = "particle-steps/s and B-S pair-evals/s vs plain FP64 GPU, at ≤1 µm position error"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, n_particles: int, n_steps: int):
+    """C restatement of the reference CPU path (FP64, OpenMP over particles) on a bounded sample."""
+    import numpy as np
+    from oracle import c_oracle, cct_oracle as O, workload_builder as WB
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    bl = WB.build_beamline(wl.designs[0])
+    mags = O.beamline_arrays(bl.magnets)
+    b = wl.beams[0]
+    st, kq, speed = WB.beam_state(b.coords[:n_particles], b.s0, b.kinetic_mev, bl)
+    dts = np.array([O.rk4_steps(wl.length, wl.footstep, s)[1] for s in speed])
+    ns = np.full(len(st), n_steps, np.int64)
+    c_oracle.track(mags, st[:2], kq[:2], dts[:2], np.ones(2, np.int64), threads=threads)  # warm
+    t0 = time.perf_counter()
+    c_oracle.track(mags, st, kq, dts, ns, threads=threads)
+    el = time.perf_counter() - t0
+    nseg = int(mags["kl"].shape[0])
+    psteps = len(st) * n_steps
+    model = open("/proc/cpuinfo").read().split("model name")[1].split("\n")[0].strip(" :\t") \
+        if os.path.exists("/proc/cpuinfo") else "unknown"
+    return {"value": psteps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{len(st)} protons x {n_steps} RK4 steps over all {nseg} segments + QS of {wl.name} "
+                      f"(C FP64 restatement of the reference CPU path, OpenMP {threads} threads, {el:.1f} s, "
+                      f"host cores visible {cores}, CPU {model})",
+            "pair_evals_per_s": psteps * 4 * nseg / el}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed tracking passes")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="agcct_section")
+    ap.add_argument("--ppl", type=int, default=0, help="points per lane (0: autotune)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison pass")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-sample", default="64x24", help="CPU baseline sample: particles x RK4 steps")
+    ap.add_argument("--roofline-launches", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from cctpy_amd import autotune, engine
+    from cctpy_amd import workloads as W
+    from cctpy_amd.magnets import device_parts
+
+    dev = engine.device()
+    wl = W.by_name(args.workload)
+    bls = [W.build_beamline(ops) for ops in wl.designs]
+    ps, npt = [], []
+    for bl, beam in zip(bls, wl.beams):
+        q = W.beam_particles(beam, bl)
+        ps += q
+        npt.append(len(q))
+    P = len(ps)
+    state = np.array([p.position.to_list() + p.velocity.to_list() for p in ps])
+    kq = np.array([p.e / p.relativistic_mass for p in ps])
+    steps_dt = [engine.rk4_steps(wl.length, wl.footstep, p.speed) for p in ps]
+    ns = np.array([s for s, _ in steps_dt], np.int64)
+    dt = np.array([d for _, d in steps_dt])
+    max_steps = int(ns.max())
+    t0 = time.perf_counter()
+    model = engine.DeviceModel([device_parts(bl).spec() for bl in bls], fp32=True)
+    torch.cuda.synchronize()
+    log(f"[bench] {wl.name}: {len(bls)} design(s), {model.nseg} segments, {P} protons, {max_steps} RK4 steps; "
+        f"windings generated on the GPU in {time.perf_counter() - t0:.2f} s")
+
+    s0 = torch.from_numpy(np.ascontiguousarray(state.T)).to(dev)
+    kq_t = torch.from_numpy(kq).to(dev)
+    dt_t = torch.from_numpy(dt).to(dev)
+    ns_t = torch.from_numpy(ns).to(dev)
+    st = torch.empty_like(s0)
+
+    tuned = {}
+    for prec in (32, 64):
+        if prec == 64 and args.no_fp64:
+            continue
+        tuned[prec] = (autotune.tune(model, npt, prec, s0[:3].contiguous()) if args.ppl == 0
+                       else {"ppl": args.ppl, "target_items": None})
+        log(f"[bench] fp{prec} shape: {tuned[prec]}")
+
+    def run(prec: int, passes: int, warmup: int):
+        plan = model.plan(npt, tuned[prec]["ppl"], tuned[prec]["target_items"])
+        scratch, partial = model.workspace(plan)
+        for _ in range(warmup):
+            st.copy_(s0)
+            model.track_tensors(plan, prec, st, kq_t, dt_t, ns_t, max_steps, scratch, partial)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(passes):
+            st.copy_(s0)
+            model.track_tensors(plan, prec, st, kq_t, dt_t, ns_t, max_steps, scratch, partial)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        return ms, st.cpu().numpy().T.copy(), plan, partial
+
+    ms32, out32, plan32, partial32 = run(32, args.steps, args.warmup)
+    particle_steps = float(ns.sum())
+    seg_of = np.repeat(model.design_nseg, npt)
+    pairs_per_pass = float(np.sum(ns * 4 * seg_of))
+    rate32 = particle_steps * args.steps / (ms32 / 1e3)
+    log(f"[bench] fp32: {ms32 / args.steps:.1f} ms/pass, {rate32:.4g} particle-steps/s")
+
+    fp64 = None
+    err = None
+    if not args.no_fp64:
+        ms64, out64, _, _ = run(64, 1, 1 if args.warmup > 0 else 0)
+        rate64 = particle_steps / (ms64 / 1e3)
+        d = np.linalg.norm(out32[:, :3] - out64[:, :3], axis=1)
+        err = {"max": float(d.max() * 1e6), "rms": float(np.sqrt(np.mean(d ** 2)) * 1e6),
+               "p99": float(np.percentile(d, 99) * 1e6)}
+        fp64 = {"particle_steps_per_s": rate64, "pair_evals_per_s": pairs_per_pass / (ms64 / 1e3),
+                "ms_per_step": ms64, "ppl": tuned[64]["ppl"]}
+        log(f"[bench] fp64: {ms64:.1f} ms/pass, {rate64:.4g} particle-steps/s; fp32 position error "
+            f"max {err['max']:.3g} um, rms {err['rms']:.3g} um")
+
+    # live timing of the dominant kernel (split-K Biot-Savart, FP32) on the stream it runs on
+    pts = torch.from_numpy(np.ascontiguousarray(out32[:, :3].T)).to(dev)
+    model.bs_partial_tensors(plan32, 32, pts, partial32)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.roofline_launches):
+        model.bs_partial_tensors(plan32, 32, pts, partial32)
+    e1.record()
+    torch.cuda.synchronize()
+    avg_ms = e0.elapsed_time(e1) / args.roofline_launches
+    flops = float(np.sum(np.asarray(npt) * model.design_nseg)) * FLOP_PER_PAIR
+    achieved = flops / (avg_ms / 1e3) / 1e12
+    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[32], "unit": "TFLOP/s",
+                "frac": achieved / PEAK_TFLOPS[32], "traffic": None,
+                "kernel": f"k_bs_partial_f32<{plan32.ppl}>", "avg_launch_ms": avg_ms,
+                "flop_per_launch": flops}
+    log(f"[bench] bs kernel: {avg_ms * 1e3:.1f} us/launch, {achieved:.2f} TFLOP/s "
+        f"({100 * roofline['frac']:.1f}% of FP32 vector peak)")
+
+    cpu = None
+    if not args.no_cpu:
+        n_p, n_s = (int(v) for v in args.cpu_sample.split("x"))
+        cpu = cpu_baseline(wl, n_p, n_s)
+        log(f"[bench] cpu baseline: {cpu['value']:.4g} particle-steps/s ({cpu['cores']} threads)")
+
+    out = {
+        "metric": METRIC,
+        "value": rate32,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 Biot-Savart terms (fp64 tile accumulation) / fp64 RK4 state",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "designs": len(bls), "segments": int(model.nseg), "protons": P,
+                   "rk4_steps": max_steps, "footstep_mm": wl.footstep * 1e3, "length_m": wl.length,
+                   "parallelism": "single GPU", "ppl": plan32.ppl, "work_items": plan32.nwork},
+        "pair_evals_per_s": pairs_per_pass * args.steps / (ms32 / 1e3),
+        "fp64_plain": fp64,
+        "speedup_vs_fp64": (rate32 / fp64["particle_steps_per_s"]) if fp64 else None,
+        "position_error_um_vs_fp64": err,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -44,6 +44,8 @@ def _load() -> ctypes.CDLL:
     lib.cctamd_pack_fp32.restype = INT
     lib.cctamd_plan.argtypes = [I64, P, P, P, P, INT, I64, I64, P, P, P]
     lib.cctamd_plan.restype = I64
+    lib.cctamd_bs_partial.argtypes = [INT, INT, P, P, P, I64, P, I64, P, P]
+    lib.cctamd_bs_partial.restype = INT
     lib.cctamd_bs_field.argtypes = [INT, INT, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P]
     lib.cctamd_bs_field.restype = INT
     lib.cctamd_track_rk4.argtypes = [INT, INT, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P, I64, P, P, P,
@@ -69,5 +71,5 @@ def check(code: int, what: str) -> None:
 
 EXPORTED = (
     "cctamd_version", "cctamd_error_string", "cctamd_winding_generate", "cctamd_winding_points",
-    "cctamd_pack_fp32", "cctamd_plan", "cctamd_bs_field", "cctamd_track_rk4",
+    "cctamd_pack_fp32", "cctamd_plan", "cctamd_bs_partial", "cctamd_bs_field", "cctamd_track_rk4",
 )

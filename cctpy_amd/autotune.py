@@ -1,0 +1,85 @@
+"""Tile-shape autotuner for the split-K Biot-Savart kernel.
+
+Candidates are (points per lane, work items per CU).  Each is timed with HIP
+events over a few launches of the real kernel on the real segment set and
+points; the fastest wins.  Results are cached per (device, precision, design
+sizes) in memory and in a JSON file (``$CCTAMD_TUNE_CACHE`` or
+``~/.cache/cctamd/tune.json``).
+"""
+from __future__ import annotations
+
+import json
+import os
+from typing import Dict, Optional, Sequence
+
+import torch
+
+from . import engine
+
+PPL = (1, 2, 4)
+ITEMS_PER_CU = (2, 4, 8, 16)
+_MEM: Dict[str, dict] = {}
+
+
+def _cache_path() -> str:
+    return os.environ.get("CCTAMD_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "cctamd",
+                                                             "tune.json"))
+
+
+def _load() -> dict:
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(key: str, val: dict) -> None:
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        data = _load()
+        data[key] = val
+        with open(path, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+    except OSError:
+        pass
+
+
+def _key(model: "engine.DeviceModel", npt: Sequence[int], precision: int) -> str:
+    props = torch.cuda.get_device_properties(engine.device())
+    return f"{props.name}|cu{props.multi_processor_count}|fp{precision}|" \
+           f"seg{list(map(int, model.design_nseg))}|pt{list(map(int, npt))}"
+
+
+def tune(model: "engine.DeviceModel", npt: Sequence[int], precision: int, points: torch.Tensor,
+         launches: int = 4, use_cache: bool = True) -> dict:
+    """Best {"ppl", "target_items"} for this model, point layout and precision."""
+    key = _key(model, npt, precision)
+    if use_cache:
+        if key in _MEM:
+            return _MEM[key]
+        disk = _load().get(key)
+        if disk:
+            _MEM[key] = disk
+            return disk
+    cus = engine.cu_count()
+    best: Optional[dict] = None
+    for ppl in PPL:
+        for per_cu in ITEMS_PER_CU:
+            plan = model.plan(npt, ppl, per_cu * cus)
+            _, partial = model.workspace(plan)
+            model.bs_partial_tensors(plan, precision, points, partial)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(launches):
+                model.bs_partial_tensors(plan, precision, points, partial)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / launches
+            if best is None or ms < best["ms"]:
+                best = {"ppl": ppl, "target_items": per_cu * cus, "ms": ms}
+    _MEM[key] = best
+    if use_cache:
+        _store(key, best)
+    return best

@@ -545,6 +545,16 @@ int64_t cctamd_plan(int64_t ndesign, const int64_t* design_ntile, const int64_t*
     return n;
 }
 
+int cctamd_bs_partial(int precision, int ppl, const void* seg, const double* tile_org, const int32_t* work,
+                      int64_t nwork, const double* points, int64_t npoints, double* partial, void* stream) {
+    if (!valid_common(precision, ppl, seg, tile_org, nwork, 1)) return CCTAMD_EINVAL;
+    if (npoints < 0) return CCTAMD_EINVAL;
+    if (nwork == 0 || npoints == 0) return CCTAMD_OK;
+    if (!work || !points || !partial) return CCTAMD_EINVAL;
+    return hip_status(launch_bs_any(precision, ppl, seg, tile_org, (const Work*)work, nwork, points, npoints, partial,
+                                    (hipStream_t)stream));
+}
+
 int cctamd_bs_field(int precision, int ppl, const void* seg, const double* tile_org, const int32_t* work,
                     int64_t nwork, const int64_t* design_wbase, const int32_t* design_nchunk,
                     const int64_t* design_pt_off, int64_t ndesign, const double* qs, const int64_t* design_qs_off,

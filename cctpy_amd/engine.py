@@ -152,7 +152,7 @@ class DeviceModel:
         self._plans: Dict[tuple, Plan] = {}
 
     # ------------------------------------------------------------- plans
-    def plan(self, design_npt: Sequence[int], ppl: int = 2, target_items: Optional[int] = None) -> Plan:
+    def plan(self, design_npt: Sequence[int], ppl: int = 2, target_items: Optional[int] = None) -> "Plan":
         npt = numpy.asarray(design_npt, numpy.int64)
         if npt.shape != (self.ndesign,):
             raise ValueError("one point count per design is required")
@@ -245,6 +245,30 @@ class DeviceModel:
                                        _stream()), "cctamd_track_rk4")
         out = soa.cpu().numpy().T.copy()
         return out, (rec.cpu().numpy().transpose(0, 2, 1).copy() if rec is not None else None)
+
+    # ------------------------------------------- device-resident variants
+    def workspace(self, plan: "Plan") -> Tuple[torch.Tensor, torch.Tensor]:
+        """(scratch [12][P], partial) device buffers for a plan."""
+        dev = device()
+        return (torch.empty((12, max(plan.npt, 1)), dtype=torch.float64, device=dev),
+                torch.empty(max(plan.nwork * 3 * 64 * plan.ppl, 1), dtype=torch.float64, device=dev))
+
+    def track_tensors(self, plan: "Plan", precision: int, state: torch.Tensor, kq: torch.Tensor, dt: torch.Tensor,
+                      nsteps: torch.Tensor, max_steps: int, scratch: torch.Tensor, partial: torch.Tensor,
+                      record: Optional[torch.Tensor] = None) -> None:
+        """In-place RK4 tracking of device-resident SoA state (6, P); nothing crosses PCIe."""
+        seg, org = self._seg(precision)
+        N.check(N.lib.cctamd_track_rk4(precision, plan.ppl, _p(seg), _p(org), _p(plan.work), plan.nwork,
+                                       _p(plan.wbase), _p(plan.nchunk), _p(plan.design_pt_off), self.ndesign,
+                                       _p(self.qs), _p(self.design_qs_off), _p(self.bconst), _p(state), plan.npt,
+                                       _p(kq), _p(dt), _p(nsteps), int(max_steps), _p(scratch), _p(partial),
+                                       _p(record), _stream()), "cctamd_track_rk4")
+
+    def bs_partial_tensors(self, plan: "Plan", precision: int, points: torch.Tensor, partial: torch.Tensor) -> None:
+        """One launch of the split-K Biot-Savart kernel on device points (3, P)."""
+        seg, org = self._seg(precision)
+        N.check(N.lib.cctamd_bs_partial(precision, plan.ppl, _p(seg), _p(org), _p(plan.work), plan.nwork,
+                                        _p(points), plan.npt, _p(partial), _stream()), "cctamd_bs_partial")
 
     def segments64(self) -> numpy.ndarray:
         """The FP64 segment rows (S, 8) copied to the host (inspection and tests)."""

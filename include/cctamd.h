@@ -102,6 +102,16 @@ int cctamd_bs_field(int precision, int ppl, const void *seg, const double *tile_
                     const double *points, int64_t npoints, double *partial, double *B, void *stream);
 
 /*
+ * The split-K Biot-Savart pass alone: partial[w][3][64*ppl] = sum over work
+ * item w's tiles for its point block (no reduction).  The building block of
+ * cctamd_bs_field / cctamd_track_rk4, exported for timing and for callers
+ * that reduce partials themselves.
+ */
+int cctamd_bs_partial(int precision, int ppl, const void *seg, const double *tile_org,
+                      const int32_t *work, int64_t nwork, const double *points, int64_t npoints,
+                      double *partial, void *stream);
+
+/*
  * RK4 tracking of D designs' particles through their own fields.
  * Replaces GPU_ACCELERATOR.track_multi_particle_beamline_for_magnet_with_multi_qs
  * (final_code/packages/gpu_accelerator.py:1912-2044) and the CPU ParticleRunner.run_only
