@@ -4,9 +4,10 @@
 A *step* is one full tracking pass of the whole synthetic beam through the
 workload's beamline (all RK4 steps; default workload ``agcct_section``:
 368,640 winding segments + 2 quads, 512 protons, 2,510 RK4 steps of 1 mm).
-The timed passes use the FP32 Biot-Savart path (per-tile re-centred FP32
-terms, FP64 accumulation, FP64 particle state); one pass of the plain FP64
-path is timed beside it and is the reference for the final-position error.
+The timed passes use the FP32 Biot-Savart tracking path (per-tile
+re-centred FP32 terms summed as K x q - C per tile, FP64 accumulation across
+tiles, FP64 particle state); one pass of the plain FP64 path is timed beside
+it and is the reference for the final-position error.
 Inputs are resident in HBM before timing starts; every pass restarts from the
 same initial state with a device-to-device copy.
 
@@ -47,7 +48,7 @@ def cpu_baseline(wl, n_particles: int, n_steps: int):
     mags = O.beamline_arrays(bl.magnets)
     b = wl.beams[0]
     st, kq, speed = WB.beam_state(b.coords[:n_particles], b.s0, b.kinetic_mev, bl)
-    dts = np.array([O.rk4_steps(wl.length, wl.footstep, s)[1] for s in speed])
+    dts = np.array([O.rk4_steps(wl.beam_length(0), wl.footstep, s)[1] for s in speed])
     ns = np.full(len(st), n_steps, np.int64)
     c_oracle.track(mags, st[:2], kq[:2], dts[:2], np.ones(2, np.int64), threads=threads)  # warm
     t0 = time.perf_counter()
@@ -98,7 +99,8 @@ def main() -> int:
     P = len(ps)
     state = np.array([p.position.to_list() + p.velocity.to_list() for p in ps])
     kq = np.array([p.e / p.relativistic_mass for p in ps])
-    steps_dt = [engine.rk4_steps(wl.length, wl.footstep, p.speed) for p in ps]
+    lengths = [wl.beam_length(i) for i, n in enumerate(npt) for _ in range(n)]
+    steps_dt = [engine.rk4_steps(L, wl.footstep, p.speed) for L, p in zip(lengths, ps)]
     ns = np.array([s for s, _ in steps_dt], np.int64)
     dt = np.array([d for _, d in steps_dt])
     max_steps = int(ns.max())
@@ -114,8 +116,9 @@ def main() -> int:
     ns_t = torch.from_numpy(ns).to(dev)
     st = torch.empty_like(s0)
 
+    FP32 = 33  # FP32 tracking kernel (K x q - C tile sums)
     tuned = {}
-    for prec in (32, 64):
+    for prec in (FP32, 64):
         if prec == 64 and args.no_fp64:
             continue
         tuned[prec] = (autotune.tune(model, npt, prec, s0[:3].contiguous()) if args.ppl == 0
@@ -139,7 +142,7 @@ def main() -> int:
         ms = e0.elapsed_time(e1)
         return ms, st.cpu().numpy().T.copy(), plan, partial
 
-    ms32, out32, plan32, partial32 = run(32, args.steps, args.warmup)
+    ms32, out32, plan32, partial32 = run(FP32, args.steps, args.warmup)
     particle_steps = float(ns.sum())
     seg_of = np.repeat(model.design_nseg, npt)
     pairs_per_pass = float(np.sum(ns * 4 * seg_of))
@@ -161,12 +164,12 @@ def main() -> int:
 
     # live timing of the dominant kernel (split-K Biot-Savart, FP32) on the stream it runs on
     pts = torch.from_numpy(np.ascontiguousarray(out32[:, :3].T)).to(dev)
-    model.bs_partial_tensors(plan32, 32, pts, partial32)
+    model.bs_partial_tensors(plan32, FP32, pts, partial32)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(args.roofline_launches):
-        model.bs_partial_tensors(plan32, 32, pts, partial32)
+        model.bs_partial_tensors(plan32, FP32, pts, partial32)
     e1.record()
     torch.cuda.synchronize()
     avg_ms = e0.elapsed_time(e1) / args.roofline_launches
@@ -174,7 +177,7 @@ def main() -> int:
     achieved = flops / (avg_ms / 1e3) / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[32], "unit": "TFLOP/s",
                 "frac": achieved / PEAK_TFLOPS[32], "traffic": None,
-                "kernel": f"k_bs_partial_f32<{plan32.ppl}>", "avg_launch_ms": avg_ms,
+                "kernel": f"k_bs_partial_f32k<{plan32.ppl}>", "avg_launch_ms": avg_ms,
                 "flop_per_launch": flops}
     log(f"[bench] bs kernel: {avg_ms * 1e3:.1f} us/launch, {achieved:.2f} TFLOP/s "
         f"({100 * roofline['frac']:.1f}% of FP32 vector peak)")

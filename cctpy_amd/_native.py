@@ -40,7 +40,7 @@ def _load() -> ctypes.CDLL:
     lib.cctamd_winding_generate.restype = INT
     lib.cctamd_winding_points.argtypes = [P, I64, P, P]
     lib.cctamd_winding_points.restype = INT
-    lib.cctamd_pack_fp32.argtypes = [P, I64, P, P, P]
+    lib.cctamd_pack_fp32.argtypes = [P, I64, P, P, P, P]
     lib.cctamd_pack_fp32.restype = INT
     lib.cctamd_plan.argtypes = [I64, P, P, P, P, INT, I64, I64, P, P, P]
     lib.cctamd_plan.restype = I64

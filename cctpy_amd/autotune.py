@@ -1,7 +1,8 @@
 """Tile-shape autotuner for the split-K Biot-Savart kernel.
 
 Candidates are (points per lane, work items per CU).  Each is timed with HIP
-events over a few launches of the real kernel on the real segment set and
+events over a few field evaluations (split-K Biot-Savart pass + fixed-order
+reduction, the per-stage cost of tracking) on the real segment set and
 points; the fastest wins.  Results are cached per (device, precision, design
 sizes) in memory and in a JSON file (``$CCTAMD_TUNE_CACHE`` or
 ``~/.cache/cctamd/tune.json``).
@@ -69,11 +70,12 @@ def tune(model: "engine.DeviceModel", npt: Sequence[int], precision: int, points
         for per_cu in ITEMS_PER_CU:
             plan = model.plan(npt, ppl, per_cu * cus)
             _, partial = model.workspace(plan)
-            model.bs_partial_tensors(plan, precision, points, partial)
+            out = torch.empty_like(points)
+            model.field_tensors(plan, precision, points, partial, out)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
             for _ in range(launches):
-                model.bs_partial_tensors(plan, precision, points, partial)
+                model.field_tensors(plan, precision, points, partial, out)
             e1.record()
             torch.cuda.synchronize()
             ms = e0.elapsed_time(e1) / launches

@@ -91,7 +91,7 @@ __global__ void k_winding_points(const double* __restrict__ params, int64_t npoi
 #pragma clang fp contract(fast)
 
 __global__ void k_pack_fp32(const double* __restrict__ seg64, int64_t nseg, float* __restrict__ seg32,
-                            double* __restrict__ tile_org) {
+                            double* __restrict__ tile_org, float* __restrict__ seg32k) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nseg) return;
     const int64_t t = i / kTile;
@@ -106,6 +106,21 @@ __global__ void k_pack_fp32(const double* __restrict__ seg64, int64_t nseg, floa
     d[5] = (float)(s[5] - o[1]);
     d[6] = (float)(s[6] - o[2]);
     d[7] = 0.0f;
+    if (seg32k) {
+        // {kl, kl x (m - c), m - c} rounded once from FP64
+        const double mx = s[4] - o[0], my = s[5] - o[1], mz = s[6] - o[2];
+        float* e = seg32k + i * 12;
+        e[0] = (float)s[0];
+        e[1] = (float)s[1];
+        e[2] = (float)s[2];
+        e[3] = (float)(s[1] * mz - s[2] * my);
+        e[4] = (float)(s[2] * mx - s[0] * mz);
+        e[5] = (float)(s[0] * my - s[1] * mx);
+        e[6] = (float)mx;
+        e[7] = (float)my;
+        e[8] = (float)mz;
+        e[9] = e[10] = e[11] = 0.0f;
+    }
     if (i % kTile == 0) {
         tile_org[t * 4 + 0] = o[0];
         tile_org[t * 4 + 1] = o[1];
@@ -115,9 +130,18 @@ __global__ void k_pack_fp32(const double* __restrict__ seg64, int64_t nseg, floa
 }
 
 // ----------------------------------------------------- Biot-Savart core
+// One work item: a point block of one design x one chunk of its tiles.
+// Partials of design d live at design_wbase[d]*3*PB and are laid out
+// [block][point][component][chunk], so that the reduction over chunks reads
+// contiguous memory.
 struct Work {
-    int32_t design, pt_start, pt_count, tile_start, tile_count, r0, r1, r2;
+    int32_t design, pt_start, pt_count, tile_start, tile_count, chunk, nchunk, block;
 };
+
+__device__ inline int64_t partial_base(const Work& w, int pb) {
+    const int64_t wbase = (int64_t)blockIdx.x - (int64_t)w.block * w.nchunk - w.chunk;
+    return wbase * 3 * pb + (int64_t)w.block * pb * 3 * w.nchunk + w.chunk;
+}
 
 template <int PPL>
 __device__ inline void load_points(const double* __restrict__ pts, int64_t ptot, int pt_start, int pt_count,
@@ -134,10 +158,11 @@ __device__ inline void load_points(const double* __restrict__ pts, int64_t ptot,
 }
 
 // The 4 waves of the workgroup hold partial sums for the same point block;
-// they are added in wave order (deterministic) and written as one partial.
+// they are added in wave order (deterministic) and written as one partial
+// per (point, component): element (i, c) at out[(i*3 + c) * nchunk].
 template <int PPL>
 __device__ inline void write_partial(double ax[PPL], double ay[PPL], double az[PPL], int wave, int lane,
-                                     double* __restrict__ out) {
+                                     double* __restrict__ out, int nchunk) {
     constexpr int PB = kWave * PPL;
     __shared__ double red[kWavesPerWG][3][PB];
 #pragma unroll
@@ -149,7 +174,7 @@ __device__ inline void write_partial(double ax[PPL], double ay[PPL], double az[P
     __syncthreads();
     for (int k = threadIdx.x; k < 3 * PB; k += kWG) {
         const int c = k / PB, i = k % PB;
-        out[k] = ((red[0][c][i] + red[1][c][i]) + red[2][c][i]) + red[3][c][i];
+        out[(int64_t)(i * 3 + c) * nchunk] = ((red[0][c][i] + red[1][c][i]) + red[2][c][i]) + red[3][c][i];
     }
 }
 
@@ -205,7 +230,68 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32(const float* __restrict_
             az[j] += (double)sz[j];
         }
     }
-    write_partial<PPL>(ax, ay, az, wave, lane, partial + (int64_t)blockIdx.x * 3 * PB);
+    write_partial<PPL>(ax, ay, az, wave, lane, partial + partial_base(w, PB), w.nchunk);
+}
+
+// FP32 tracking variant: kl x (q - m) = kl x q - kl x m, so within a tile the
+// sums K = sum w kl and C = sum w (kl x m) (w = |q - m|^-3) give the tile field
+// K x q - C with 14 VALU operations per pair instead of 17.  Points must stay
+// well away from the conductors (a tile spans a few cm): true inside a bore.
+template <int PPL>
+__global__ __launch_bounds__(kWG) void k_bs_partial_f32k(const float* __restrict__ seg32k,
+                                                          const double* __restrict__ tile_org,
+                                                          const Work* __restrict__ work,
+                                                          const double* __restrict__ pts, int64_t ptot,
+                                                          double* __restrict__ partial) {
+    constexpr int PB = kWave * PPL;
+    const Work w = work[blockIdx.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double px[PPL], py[PPL], pz[PPL];
+    load_points<PPL>(pts, ptot, w.pt_start, w.pt_count, lane, px, py, pz);
+    double ax[PPL], ay[PPL], az[PPL];
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) ax[j] = ay[j] = az[j] = 0.0;
+
+    const int tend = w.tile_start + w.tile_count;
+    for (int t = w.tile_start + wave; t < tend; t += kWavesPerWG) {
+        const double ox = tile_org[4 * t], oy = tile_org[4 * t + 1], oz = tile_org[4 * t + 2];
+        float qx[PPL], qy[PPL], qz[PPL], kx[PPL], ky[PPL], kz[PPL], cx[PPL], cy[PPL], cz[PPL];
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            qx[j] = (float)(px[j] - ox);
+            qy[j] = (float)(py[j] - oy);
+            qz[j] = (float)(pz[j] - oz);
+            kx[j] = ky[j] = kz[j] = cx[j] = cy[j] = cz[j] = 0.0f;
+        }
+        const float4* s = reinterpret_cast<const float4*>(seg32k) + (int64_t)t * kTile * 3;
+#pragma unroll 8
+        for (int k = 0; k < kTile; ++k) {
+            const float4 a = s[3 * k];      // kl.xyz, (kl x m).x
+            const float4 b = s[3 * k + 1];  // (kl x m).yz, m.xy
+            const float4 c = s[3 * k + 2];  // m.z
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                const float rx = qx[j] - b.z, ry = qy[j] - b.w, rz = qz[j] - c.x;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float w3 = inv * inv * inv;
+                kx[j] += a.x * w3;
+                ky[j] += a.y * w3;
+                kz[j] += a.z * w3;
+                cx[j] += a.w * w3;
+                cy[j] += b.x * w3;
+                cz[j] += b.y * w3;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            ax[j] += (double)((ky[j] * qz[j] - kz[j] * qy[j]) - cx[j]);
+            ay[j] += (double)((kz[j] * qx[j] - kx[j] * qz[j]) - cy[j]);
+            az[j] += (double)((kx[j] * qy[j] - ky[j] * qx[j]) - cz[j]);
+        }
+    }
+    write_partial<PPL>(ax, ay, az, wave, lane, partial + partial_base(w, PB), w.nchunk);
 }
 
 // Plain FP64 terms on global-frame segments.
@@ -242,7 +328,7 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f64(const double* __restrict
             }
         }
     }
-    write_partial<PPL>(ax, ay, az, wave, lane, partial + (int64_t)blockIdx.x * 3 * PB);
+    write_partial<PPL>(ax, ay, az, wave, lane, partial + partial_base(w, PB), w.nchunk);
 }
 
 // ------------------------------------------------ reduction + QS + RK4
@@ -273,27 +359,35 @@ __device__ inline int64_t find_design(const int64_t* __restrict__ off, int64_t n
     return lo;
 }
 
-// Sum of the chunk partials of point g (one wave per point, fixed lane order,
-// butterfly reduction: deterministic), then QS and uniform contributions.
-__device__ inline bool point_field(int64_t g, int ppl, const double* __restrict__ partial,
-                                   const int64_t* __restrict__ design_wbase, const int32_t* __restrict__ design_nchunk,
-                                   const int64_t* __restrict__ design_pt_off, int64_t nd,
-                                   const double* __restrict__ qs, const int64_t* __restrict__ design_qs_off,
-                                   const double* __restrict__ bconst, const double p[3], double b[3]) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t d = find_design(design_pt_off, nd, g);
+// Location of point g's partials: design, then the [component][chunk] rows.
+struct PointSlot {
+    int64_t design;
+    const double* rows;  // rows[c * nc + chunk] for component c
+    int nc;
+};
+
+__device__ inline PointSlot point_slot(int64_t g, int ppl, const double* __restrict__ partial,
+                                       const int64_t* __restrict__ design_wbase,
+                                       const int32_t* __restrict__ design_nchunk,
+                                       const int64_t* __restrict__ design_pt_off, int64_t nd) {
+    PointSlot s;
+    s.design = find_design(design_pt_off, nd, g);
     const int pb = kWave * ppl;
-    const int64_t lp = g - design_pt_off[d];
-    const int64_t blk = lp / pb;
-    const int i = (int)(lp % pb);
-    const int nc = design_nchunk[d];
-    const int64_t base = design_wbase[d] + blk * nc;
+    const int64_t lp = g - design_pt_off[s.design];
+    s.nc = design_nchunk[s.design];
+    s.rows = partial + design_wbase[s.design] * 3 * pb + ((lp / pb) * pb + lp % pb) * 3 * (int64_t)s.nc;
+    return s;
+}
+
+// Fixed-order sum over chunks (lanes stride the chunks, then a butterfly):
+// deterministic, and every lane ends with the total.
+__device__ inline void chunk_sum(const PointSlot& s, double b[3]) {
+    const int lane = threadIdx.x & (kWave - 1);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int c = lane; c < nc; c += kWave) {
-        const double* pr = partial + (base + c) * 3 * pb;
-        s0 += pr[i];
-        s1 += pr[pb + i];
-        s2 += pr[2 * pb + i];
+    for (int c = lane; c < s.nc; c += kWave) {
+        s0 += s.rows[c];
+        s1 += s.rows[s.nc + c];
+        s2 += s.rows[2 * s.nc + c];
     }
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -301,15 +395,18 @@ __device__ inline bool point_field(int64_t g, int ppl, const double* __restrict_
         s1 += __shfl_xor(s1, off);
         s2 += __shfl_xor(s2, off);
     }
-    if (lane != 0) return false;
     b[0] = s0;
     b[1] = s1;
     b[2] = s2;
+}
+
+__device__ inline void analytic_fields(int64_t d, const double* __restrict__ qs,
+                                       const int64_t* __restrict__ design_qs_off,
+                                       const double* __restrict__ bconst, const double p[3], double b[3]) {
     for (int64_t k = design_qs_off[d]; k < design_qs_off[d + 1]; ++k) qs_add(qs + 16 * k, p, b);
     b[0] = b[0] + bconst[4 * d];
     b[1] = b[1] + bconst[4 * d + 1];
     b[2] = b[2] + bconst[4 * d + 2];
-    return true;
 }
 
 __global__ __launch_bounds__(kWG) void k_field_reduce(int ppl, const double* __restrict__ partial,
@@ -324,12 +421,14 @@ __global__ __launch_bounds__(kWG) void k_field_reduce(int ppl, const double* __r
     const int64_t g = (int64_t)blockIdx.x * kWavesPerWG + (threadIdx.x >> 6);
     if (g >= ptot) return;
     const double p[3] = {pts[g], pts[ptot + g], pts[2 * ptot + g]};
+    const PointSlot sl = point_slot(g, ppl, partial, design_wbase, design_nchunk, design_pt_off, nd);
     double b[3];
-    if (point_field(g, ppl, partial, design_wbase, design_nchunk, design_pt_off, nd, qs, design_qs_off, bconst, p, b)) {
-        B[g] = b[0];
-        B[ptot + g] = b[1];
-        B[2 * ptot + g] = b[2];
-    }
+    chunk_sum(sl, b);
+    if ((threadIdx.x & (kWave - 1)) != 0) return;
+    analytic_fields(sl.design, qs, design_qs_off, bconst, p, b);
+    B[g] = b[0];
+    B[ptot + g] = b[1];
+    B[2 * ptot + g] = b[2];
 }
 
 // scratch rows: 0-2 stage position (read by k_bs_partial), 3-5 stage
@@ -362,43 +461,56 @@ __global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __rest
                                                    double* __restrict__ record) {
     const int64_t g = (int64_t)blockIdx.x * kWavesPerWG + (threadIdx.x >> 6);
     if (g >= ptot) return;
-    if (step >= nsteps[g]) return;  // finished particles keep their state
     double* ps = scratch;
     double* vs = scratch + 3 * ptot;
     double* kp = scratch + 6 * ptot;
     double* kv = scratch + 9 * ptot;
-    const double p[3] = {ps[g], ps[ptot + g], ps[2 * ptot + g]};
-    double b[3];
-    if (!point_field(g, ppl, partial, design_wbase, design_nchunk, design_pt_off, nd, qs, design_qs_off, bconst, p, b))
-        return;
-    const double v[3] = {vs[g], vs[ptot + g], vs[2 * ptot + g]};
+    // every load that does not depend on the partial sums is issued first
+    const bool active = step < nsteps[g];
     const double k = kq[g];
+    const double h = dt[g];
+    double p[3], v[3], p0[3], v0[3], sp[3], sv[3];
+    for (int c = 0; c < 3; ++c) {
+        const int64_t o = c * ptot + g;
+        p[c] = ps[o];
+        v[c] = vs[o];
+        p0[c] = state[o];
+        v0[c] = state[3 * ptot + o];
+        if (S != 0) {
+            sp[c] = kp[o];
+            sv[c] = kv[o];
+        }
+    }
+    const PointSlot sl = point_slot(g, ppl, partial, design_wbase, design_nchunk, design_pt_off, nd);
+    if (!active) return;  // finished particles keep their state
+    double b[3];
+    chunk_sum(sl, b);
+    if ((threadIdx.x & (kWave - 1)) != 0) return;
+    analytic_fields(sl.design, qs, design_qs_off, bconst, p, b);
     const double cr[3] = {v[1] * b[2] - v[2] * b[1], -v[0] * b[2] + v[2] * b[0], v[0] * b[1] - v[1] * b[0]};
     const double a[3] = {cr[0] * k, cr[1] * k, cr[2] * k};
-    const double h = dt[g];
     const double h2 = h / 2;
     for (int c = 0; c < 3; ++c) {
         const int64_t o = c * ptot + g;
-        const double p0 = state[o], v0 = state[3 * ptot + o];
         if (S == 0) {
             kp[o] = v[c];
             kv[o] = a[c];
-            ps[o] = p0 + v[c] * h2;
-            vs[o] = v0 + a[c] * h2;
+            ps[o] = p0[c] + v[c] * h2;
+            vs[o] = v0[c] + a[c] * h2;
         } else if (S == 1) {
-            kp[o] = kp[o] + v[c] * 2;
-            kv[o] = kv[o] + a[c] * 2;
-            ps[o] = p0 + v[c] * h2;
-            vs[o] = v0 + a[c] * h2;
+            kp[o] = sp[c] + v[c] * 2;
+            kv[o] = sv[c] + a[c] * 2;
+            ps[o] = p0[c] + v[c] * h2;
+            vs[o] = v0[c] + a[c] * h2;
         } else if (S == 2) {
-            kp[o] = kp[o] + v[c] * 2;
-            kv[o] = kv[o] + a[c] * 2;
-            ps[o] = p0 + v[c] * h;
-            vs[o] = v0 + a[c] * h;
+            kp[o] = sp[c] + v[c] * 2;
+            kv[o] = sv[c] + a[c] * 2;
+            ps[o] = p0[c] + v[c] * h;
+            vs[o] = v0[c] + a[c] * h;
         } else {
             const double h6 = h / 6;
-            const double np = p0 + (kp[o] + v[c]) * h6;
-            const double nv = v0 + (kv[o] + a[c]) * h6;
+            const double np = p0[c] + (sp[c] + v[c]) * h6;
+            const double nv = v0[c] + (sv[c] + a[c]) * h6;
             state[o] = np;
             state[3 * ptot + o] = nv;
             ps[o] = np;
@@ -425,6 +537,9 @@ hipError_t launch_bs(int precision, const void* seg, const double* tile_org, con
     if (precision == 32)
         hipLaunchKernelGGL(k_bs_partial_f32<PPL>, dim3((unsigned)nwork), dim3(kWG), 0, st,
                            (const float*)seg, tile_org, work, pts, ptot, partial);
+    else if (precision == 33)
+        hipLaunchKernelGGL(k_bs_partial_f32k<PPL>, dim3((unsigned)nwork), dim3(kWG), 0, st,
+                           (const float*)seg, tile_org, work, pts, ptot, partial);
     else
         hipLaunchKernelGGL(k_bs_partial_f64<PPL>, dim3((unsigned)nwork), dim3(kWG), 0, st,
                            (const double*)seg, work, pts, ptot, partial);
@@ -442,11 +557,11 @@ hipError_t launch_bs_any(int precision, int ppl, const void* seg, const double* 
 
 bool valid_common(int precision, int ppl, const void* seg, const double* tile_org, int64_t nwork,
                   int64_t ndesign) {
-    if (precision != 32 && precision != 64) return false;
+    if (precision != 32 && precision != 33 && precision != 64) return false;
     if (ppl != 1 && ppl != 2 && ppl != 4) return false;
     if (nwork < 0 || ndesign <= 0 || nwork > 0x7fffffff) return false;
     if (nwork > 0 && seg == nullptr) return false;
-    if (nwork > 0 && precision == 32 && tile_org == nullptr) return false;
+    if (nwork > 0 && precision != 64 && tile_org == nullptr) return false;
     return true;
 }
 
@@ -488,7 +603,8 @@ int cctamd_winding_points(const double* params, int64_t npoints, double* out, vo
     return hip_status(hipGetLastError());
 }
 
-int cctamd_pack_fp32(const double* seg64, int64_t nseg, float* seg32, double* tile_org, void* stream) {
+int cctamd_pack_fp32(const double* seg64, int64_t nseg, float* seg32, double* tile_org, float* seg32k,
+                     void* stream) {
     if (nseg < 0 || nseg % kTile) return CCTAMD_EINVAL;
     if (nseg == 0) return CCTAMD_OK;
     if (!seg64 || !seg32 || !tile_org) return CCTAMD_EINVAL;
@@ -496,7 +612,7 @@ int cctamd_pack_fp32(const double* seg64, int64_t nseg, float* seg32, double* ti
     const int64_t blocks = ceil_div(nseg, threads);
     if (blocks > 0x7fffffff) return CCTAMD_ERANGE;
     hipLaunchKernelGGL(k_pack_fp32, dim3((unsigned)blocks), dim3(threads), 0, (hipStream_t)stream, seg64, nseg,
-                       seg32, tile_org);
+                       seg32, tile_org, seg32k);
     return hip_status(hipGetLastError());
 }
 
@@ -536,7 +652,9 @@ int64_t cctamd_plan(int64_t ndesign, const int64_t* design_ntile, const int64_t*
                     w[2] = (int32_t)pc;
                     w[3] = (int32_t)ts;
                     w[4] = (int32_t)(t1 - t0);
-                    w[5] = w[6] = w[7] = 0;
+                    w[5] = (int32_t)c;
+                    w[6] = (int32_t)nc;
+                    w[7] = (int32_t)b;
                 }
             }
         }

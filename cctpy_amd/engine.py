@@ -133,12 +133,13 @@ class DeviceModel:
             N.check(N.lib.cctamd_winding_generate(_p(prm), _p(off), _p(dsts), self.ncct, int(sum(counts)),
                                                   _p(seg64), _stream()), "cctamd_winding_generate")
         self.seg64 = seg64
-        self.seg32 = self.tile_org = None
+        self.seg32 = self.seg32k = self.tile_org = None
         if fp32:
             self.seg32 = torch.empty((max(S, 1), 8), dtype=torch.float32, device=dev)
+            self.seg32k = torch.empty((max(S, 1), 12), dtype=torch.float32, device=dev)
             self.tile_org = torch.empty((max(S // N.TILE, 1), 4), dtype=torch.float64, device=dev)
-            N.check(N.lib.cctamd_pack_fp32(_p(seg64), S, _p(self.seg32), _p(self.tile_org), _stream()),
-                    "cctamd_pack_fp32")
+            N.check(N.lib.cctamd_pack_fp32(_p(seg64), S, _p(self.seg32), _p(self.tile_org), _p(self.seg32k),
+                                           _stream()), "cctamd_pack_fp32")
         qs = [s.qs for s in self.specs]
         nq = numpy.array([q.shape[0] for q in qs], numpy.int64)
         self.design_qs_off_host = numpy.concatenate([[0], numpy.cumsum(nq)]).astype(numpy.int64)
@@ -185,11 +186,11 @@ class DeviceModel:
     def _seg(self, precision: int):
         if precision == 64:
             return self.seg64, None
-        if precision == 32:
+        if precision in (32, 33):
             if self.seg32 is None:
                 raise ValueError("model was built without FP32 tiles")
-            return self.seg32, self.tile_org
-        raise ValueError("precision must be 32 or 64")
+            return (self.seg32 if precision == 32 else self.seg32k), self.tile_org
+        raise ValueError("precision must be 32, 33 or 64")
 
     # ------------------------------------------------------------- field
     def field(self, points: numpy.ndarray, design_npt: Sequence[int], precision: int = 64, ppl: int = 2) -> numpy.ndarray:
@@ -263,6 +264,15 @@ class DeviceModel:
                                        _p(self.qs), _p(self.design_qs_off), _p(self.bconst), _p(state), plan.npt,
                                        _p(kq), _p(dt), _p(nsteps), int(max_steps), _p(scratch), _p(partial),
                                        _p(record), _stream()), "cctamd_track_rk4")
+
+    def field_tensors(self, plan: "Plan", precision: int, points: torch.Tensor, partial: torch.Tensor,
+                      out: torch.Tensor) -> None:
+        """B (3, P) at device points (3, P): Biot-Savart pass + fixed-order reduction + QS + uniform."""
+        seg, org = self._seg(precision)
+        N.check(N.lib.cctamd_bs_field(precision, plan.ppl, _p(seg), _p(org), _p(plan.work), plan.nwork,
+                                      _p(plan.wbase), _p(plan.nchunk), _p(plan.design_pt_off), self.ndesign,
+                                      _p(self.qs), _p(self.design_qs_off), _p(self.bconst), _p(points), plan.npt,
+                                      _p(partial), _p(out), _stream()), "cctamd_bs_field")
 
     def bs_partial_tensors(self, plan: "Plan", precision: int, points: torch.Tensor, partial: torch.Tensor) -> None:
         """One launch of the split-K Biot-Savart kernel on device points (3, P)."""

@@ -87,8 +87,13 @@ class GPU_ACCELERATOR:
         return self._field(bl, p)
 
     # ---------------------------------------------------------- tracking
+    @property
+    def _track_precision(self) -> int:
+        # FP32 tracking uses the K x q - C tile sums (precision 33 of the C ABI)
+        return 33 if self.precision == 32 else 64
+
     def _track_inplace(self, bl: Magnet, ps: List[RunningParticle], distance: float, footstep: float) -> None:
-        out, _, _, _ = track_particles(ps, [bl], [0] * len(ps), distance, footstep, precision=self.precision)
+        out, _, _, _ = track_particles(ps, [bl], [0] * len(ps), distance, footstep, precision=self._track_precision)
         for q, row in zip(ps, out):
             q.position = P3(*row[:3])
             q.velocity = P3(*row[3:])
@@ -122,7 +127,8 @@ class GPU_ACCELERATOR:
         [beamline][particle], inputs untouched (gpu_accelerator.py:1912-2044)."""
         allp = [q for _ in bls for q in ps]
         design_of = [d for d in range(len(bls)) for _ in ps]
-        out, _, _, _ = track_particles(allp, list(bls), design_of, distance, footstep, precision=self.precision)
+        out, _, _, _ = track_particles(allp, list(bls), design_of, distance, footstep,
+                                       precision=self._track_precision)
         ret = []
         for d in range(len(bls)):
             row = []

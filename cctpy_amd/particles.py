@@ -180,7 +180,7 @@ def track_particles(ps: Sequence[RunningParticle], magnets: Sequence[Magnet], de
         state[row, 3:] = p.velocity.to_list()
         kq[row] = p.e / p.relativistic_mass
         ns[row], dt[row] = engine.rk4_steps(length, footstep, p.speed)
-    model = engine.model_for(specs, fp32=(precision == 32))
+    model = engine.model_for(specs, fp32=(precision != 64))
     out, rec = model.track(state, kq, dt, ns, npt, precision=precision, ppl=ppl, record=record)
     inv = numpy.empty(len(ps), numpy.int64)
     inv[numpy.asarray(order, numpy.int64)] = numpy.arange(len(ps))

@@ -22,8 +22,13 @@ natural frame at the start):
   (2.509 m); 512 protons, same transverse beam, delta ~ U(+-8%), 1 mm steps.
 * ``population``   64 designs of the dipole-pair/AG form: layers in {2, 4},
   windings in [100, 160], {360, 720} segments per winding, currents in
-  [8000, 10000] A, 4 seeded tilts; 72,000 - 460,800 segments each; 32 protons
-  per design (single_cct beam), 1 mm steps over a 30 deg arc.
+  [8000, 10000] A (AG layers -2000 to -4000 A), tilts [30, t1, t2, t3] with
+  harmonic tilts seeded in [70, 110] deg (wider ranges put multipoles as strong
+  as the dipole into the dipole layers and the beam into the coils); 72,000 -
+  460,800 segments each; each
+  design bends by the angle its dipole field sustains for 250 MeV protons on
+  R = 0.95 m (41-103 deg) and its 32 protons (single_cct beam) are tracked
+  along that arc in 1 mm steps.
 * ``hust_total``   the full two-section HUST gantry at 720 segments per winding
   (959,040 segments); 4096 protons, delta ~ U(+-8%), 0.5 mm steps over 6 m.
 """
@@ -43,10 +48,12 @@ Op = Tuple[str, Dict]
 
 @dataclass
 class Beam:
-    """Phase-space coordinates (n, 5): x, xp, y, yp, delta; ideal particle at s0 with kinetic energy."""
+    """Phase-space coordinates (n, 5): x, xp, y, yp, delta; ideal particle at s0 with kinetic energy;
+    ``length`` overrides the workload's tracking length for this beam."""
     coords: numpy.ndarray
     s0: float
     kinetic_mev: float
+    length: float = None
 
 
 @dataclass
@@ -61,6 +68,10 @@ class Workload:
     def n_particles(self) -> int:
         return sum(b.coords.shape[0] for b in self.beams)
 
+    def beam_length(self, i: int) -> float:
+        b = self.beams[i]
+        return self.length if b.length is None else b.length
+
 
 def gaussian_beam(rng, n: int, delta: float) -> numpy.ndarray:
     c = numpy.empty((n, 5))
@@ -72,8 +83,8 @@ def gaussian_beam(rng, n: int, delta: float) -> numpy.ndarray:
     return c
 
 
-def _tilt(rng) -> float:
-    t = float(rng.uniform(20.0, 160.0))
+def _tilt(rng, lo: float = 20.0, hi: float = 160.0) -> float:
+    t = float(rng.uniform(lo, hi))
     return t if abs(t - 90.0) > 1e-3 else 90.5
 
 
@@ -112,31 +123,37 @@ def agcct_section(n_particles: int = 512, seed: int = SEEDS["agcct_section"],
 
 
 def population(n_designs: int = 64, n_particles: int = 32, seed: int = SEEDS["population"]) -> Workload:
+    """Bend angle of each design from its own dipole strength: a CCT dipole layer pair gives
+    B = mu0 I cot(t) / pitch with pitch = R theta / W, so theta = mu0 I cot(t) W / (R B_ref)
+    with B_ref = p/(qR) of 250 MeV protons; each design is tracked along its own arc."""
     rng = numpy.random.default_rng(seed)
     designs, beams = [], []
-    arc = 0.95 * math.radians(30.0)
+    big_r, mu0 = 0.95, 4e-7 * math.pi
+    b_ref = 2.5601351895878315  # 250 MeV proton rigidity / 0.95 m
     for _ in range(n_designs):
         layers = int(rng.choice([2, 4]))
         w = int(rng.integers(100, 161))
         n = int(rng.choice([360, 720]))
         i_dip = float(rng.uniform(8000.0, 10000.0))
-        tilts = [30.0, _tilt(rng), _tilt(rng), _tilt(rng)]
+        tilts = [30.0, _tilt(rng, 70.0, 110.0), _tilt(rng, 70.0, 110.0), _tilt(rng, 70.0, 110.0)]
+        theta = round(math.degrees(mu0 * i_dip / math.tan(math.radians(30.0)) * w / (big_r * b_ref)), 1)
         ops: List[Op] = [("first_drift", dict(length=0.1))]
         if layers == 2:
-            ops.append(("append_dipole_cct", dict(big_r=0.95, small_r_inner=95 * MM, small_r_outer=107 * MM,
-                                                  bending_angle=-30.0, tilt_angles=tilts, winding_number=w,
+            ops.append(("append_dipole_cct", dict(big_r=big_r, small_r_inner=95 * MM, small_r_outer=107 * MM,
+                                                  bending_angle=-theta, tilt_angles=tilts, winding_number=w,
                                                   current=i_dip, disperse_number_per_winding=n)))
         else:
             w1 = w // 3
-            ops.append(("append_agcct", dict(big_r=0.95, small_rs=[119 * MM, 107 * MM, 95 * MM, 83 * MM],
-                                             bending_angles=[-10.0, -20.0], tilt_angles=[tilts, [90.0, 30.0]],
+            ops.append(("append_agcct", dict(big_r=big_r, small_rs=[119 * MM, 107 * MM, 95 * MM, 83 * MM],
+                                             bending_angles=[-theta * w1 / w, -theta * (w - w1) / w],
+                                             tilt_angles=[tilts, [90.0, 30.0]],
                                              winding_numbers=[[w], [w1, w - w1]],
-                                             currents=[i_dip, -float(rng.uniform(8000.0, 10000.0))],
+                                             currents=[i_dip, -float(rng.uniform(2000.0, 4000.0))],
                                              disperse_number_per_winding=n)))
         ops.append(("append_drift", dict(length=0.1)))
         designs.append(ops)
-        beams.append(Beam(gaussian_beam(rng, n_particles, 0.02), 0.1, 250.0))
-    return Workload("population", designs, beams, arc, 1 * MM)
+        beams.append(Beam(gaussian_beam(rng, n_particles, 0.02), 0.1, 250.0, big_r * math.radians(theta)))
+    return Workload("population", designs, beams, max(b.length for b in beams), 1 * MM)
 
 
 def hust_total(n_particles: int = 4096, seed: int = SEEDS["hust_total"], length: float = 6.0,

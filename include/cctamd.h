@@ -68,15 +68,20 @@ int cctamd_winding_generate(const double *params, const int64_t *count_off, cons
  */
 int cctamd_winding_points(const double *params, int64_t npoints, double *out, void *stream);
 
-/* FP32 tile packing of a FP64 segment set (nseg a multiple of CCTAMD_TILE). */
-int cctamd_pack_fp32(const double *seg64, int64_t nseg, float *seg32, double *tile_org, void *stream);
+/*
+ * FP32 tile packing of a FP64 segment set (nseg a multiple of CCTAMD_TILE).
+ * seg32k (nullable) receives the 12-float rows {kl, kl x (m-c), m-c, 0, 0, 0} of
+ * precision 33 (see cctamd_bs_field).
+ */
+int cctamd_pack_fp32(const double *seg64, int64_t nseg, float *seg32, double *tile_org, float *seg32k,
+                     void *stream);
 
 /*
  * Host-only planner: splits (design, point block, tile chunk) work.
  *   design_ntile[d], design_npt[d], design_tile_off[d], design_pt_off[d] (host).
  *   ppl: points per lane (1, 2 or 4); a point block holds 64*ppl points.
  * With work == NULL returns the number of work items only. Otherwise fills
- * work[nwork][8] = {design, pt_start, pt_count, tile_start, tile_count, 0, 0, 0},
+ * work[nwork][8] = {design, pt_start, pt_count, tile_start, tile_count, chunk, nchunk, block},
  * design_wbase[d] (first item of design d) and design_nchunk[d].
  * Returns the number of items, or a negative code.
  */
@@ -91,9 +96,12 @@ int64_t cctamd_plan(int64_t ndesign, const int64_t *design_ntile, const int64_t 
  * magnet_at_beamline_with_single_qs (final_code/packages/gpu_accelerator.py:1147-1196,
  * 1287-1347, 1349-1431) and CCT/Beamline.magnetic_field_at (cct.py:249-285,
  * beamline.py:46-53).
- *   precision: 64 (seg64, FP64 sum) or 32 (seg32 + tile_org, FP32 terms,
- *              FP64 accumulation across tiles).
- *   partial: device workspace of nwork*3*64*ppl doubles.
+ *   precision: 64 (seg64, FP64 sum), 32 (seg32 + tile_org: FP32 terms of the
+ *              re-centred tile, FP64 accumulation across tiles) or 33 (seg32k +
+ *              tile_org: the same with per-tile sums K = sum w kl, C = sum w kl x m and
+ *              B_tile = K x q - C; for points away from the windings, e.g. in a bore).
+ *   partial: device workspace of nwork*3*64*ppl doubles; design d's partials start at
+ *            design_wbase[d]*3*64*ppl, laid out [block][point][component][chunk].
  */
 int cctamd_bs_field(int precision, int ppl, const void *seg, const double *tile_org,
                     const int32_t *work, int64_t nwork, const int64_t *design_wbase,
@@ -102,8 +110,9 @@ int cctamd_bs_field(int precision, int ppl, const void *seg, const double *tile_
                     const double *points, int64_t npoints, double *partial, double *B, void *stream);
 
 /*
- * The split-K Biot-Savart pass alone: partial[w][3][64*ppl] = sum over work
- * item w's tiles for its point block (no reduction).  The building block of
+ * The split-K Biot-Savart pass alone: for every work item, the sum over its
+ * tiles for each point of its block, written in the partial layout of
+ * cctamd_bs_field (no reduction).  The building block of
  * cctamd_bs_field / cctamd_track_rk4, exported for timing and for callers
  * that reduce partials themselves.
  */

@@ -88,6 +88,28 @@ def test_field_fp32_recentred_accuracy():
     assert err.max() < 1e-6, err.max()
 
 
+def test_field_fp32_kc_in_bore_accuracy():
+    """Precision 33 (tile sums K x q - C) inside the bore of a 720-segment/winding AG-CCT section."""
+    from cctpy_amd import workloads as W
+    from oracle import workload_builder as WB
+    wl = W.agcct_section(n_particles=8)
+    bl = W.build_beamline(wl.designs[0])
+    ref = WB.build_beamline(wl.designs[0])
+    rng = np.random.default_rng(5)
+    pts = []
+    for s in rng.uniform(0.3, bl.get_length() - 0.3, 200):
+        p, d = ref.traj.point_at(float(s)), ref.traj.direct_at(float(s))
+        dx, dz = rng.uniform(-0.03, 0.03), rng.uniform(-0.03, 0.03)
+        pts.append((p[0] - d[1] * dx, p[1] + d[0] * dx, dz))
+    pts = np.array(pts)
+    model = engine.model_for([device_parts(bl).spec()])
+    want = C.field(O.beamline_arrays(ref.magnets), pts)
+    for prec in (32, 33):
+        b = model.field(pts, [len(pts)], precision=prec)
+        err = np.linalg.norm(b - want, axis=1) / np.linalg.norm(want, axis=1)
+        assert err.max() < 1e-6, (prec, err.max())
+
+
 def test_qs_fields_bitwise(golden):
     for key in ("qs_a13", "qs_c05"):
         g = golden[key]

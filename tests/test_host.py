@@ -59,6 +59,7 @@ def test_plan_covers_every_point_tile_pair_once(ppl):
             for c in range(nch[d]):
                 w = work[wbase[d] + b * nch[d] + c]
                 assert w[0] == d and w[1] == poff[d] + b * pb and w[2] == min(pb, npt[d] - b * pb)
+                assert (w[5], w[6], w[7]) == (c, nch[d], b)
                 for t in range(w[3], w[3] + w[4]):
                     for q in range(w[1], w[1] + w[2]):
                         cover[(t, q)] = cover.get((t, q), 0) + 1
