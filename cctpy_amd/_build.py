@@ -6,7 +6,7 @@ import shutil
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "cctamd.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("cctamd.hip", "adjoint.hip")]
 LIB = os.path.join(HERE, "libcctamd.so")
 ARCH = "gfx950"
 
@@ -29,7 +29,7 @@ def hipcc() -> str:
 def build(verbose: bool = False) -> str:
     """Compile the library (always recompiles) and return its path."""
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, SRC]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *SRCS]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + res.stdout + res.stderr)

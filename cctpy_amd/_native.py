@@ -49,8 +49,13 @@ def _load() -> ctypes.CDLL:
     lib.cctamd_bs_field.argtypes = [INT, INT, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P]
     lib.cctamd_bs_field.restype = INT
     lib.cctamd_track_rk4.argtypes = [INT, INT, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P, I64, P, P, P,
-                                     P]
+                                     I64, P]
     lib.cctamd_track_rk4.restype = INT
+    lib.cctamd_winding_tangents.argtypes = [P, P, P, I64, I64, P, P]
+    lib.cctamd_winding_tangents.restype = INT
+    lib.cctamd_track_rk4_adjoint.argtypes = [INT, INT, P, P, P, P, P, P, I64, P, I64, P, P, P, I64, P, P, P,
+                                             P, I64, I64, P, P, P, I64, P, P, P, P, P, P, P, P, P, P, P]
+    lib.cctamd_track_rk4_adjoint.restype = INT
     return lib
 
 
@@ -72,4 +77,5 @@ def check(code: int, what: str) -> None:
 EXPORTED = (
     "cctamd_version", "cctamd_error_string", "cctamd_winding_generate", "cctamd_winding_points",
     "cctamd_pack_fp32", "cctamd_plan", "cctamd_bs_partial", "cctamd_bs_field", "cctamd_track_rk4",
+    "cctamd_winding_tangents", "cctamd_track_rk4_adjoint",
 )

@@ -1,0 +1,200 @@
+"""Gradients of tracking results with respect to magnet parameters (reverse mode on the GPU).
+
+``AdjointTracker`` runs the forward RK4 tracking with checkpoints every K
+steps and the discrete adjoint (``cctamd_track_rk4_adjoint``: forward
+recompute per checkpoint interval, stage adjoints, backward Biot-Savart
+kernel).  It returns dJ/d(initial state), per-CCT dJ/d(current) and
+dJ/d(tilt coefficients), and per-QS dJ/d(gradient), dJ/d(second gradient).
+
+``DesignProblem`` chains these to a parameter vector theta of a beamline
+builder (tilt angles, currents, QS gradients, ...) and ``track_fn`` exposes
+theta -> final particle states as a ``torch.autograd.Function``.
+"""
+from __future__ import annotations
+
+import math
+from typing import Callable, List, Optional, Sequence, Tuple
+
+import numpy
+import torch
+
+from . import _native as N
+from . import engine
+from .engine import _p, _stream
+from .magnets import Magnet, device_parts
+
+
+class AdjointTracker:
+    """Forward tracking with checkpoints and its adjoint for one set of designs."""
+
+    def __init__(self, magnets: Sequence[Magnet], precision: int = 33, ckpt_every: int = 50, ppl: int = 2):
+        specs = []
+        for m in magnets:
+            parts = device_parts(m)
+            if parts.foreign:
+                raise ValueError(f"magnet {parts.foreign[0]} cannot be tracked on the GPU")
+            specs.append(parts.spec())
+        self.specs = specs
+        self.model = engine.DeviceModel(specs, fp32=True, align_cct=True)
+        self.precision = precision
+        self.ckpt_every = int(ckpt_every)
+        self.ppl = ppl
+
+    def forward(self, state: numpy.ndarray, kq: numpy.ndarray, dt: numpy.ndarray, nsteps: numpy.ndarray,
+                design_npt: Sequence[int]) -> numpy.ndarray:
+        """Track (P, 6) states; keeps the checkpoints for ``backward``."""
+        dev = engine.device()
+        m = self.model
+        self.plan = plan = m.plan(design_npt, self.ppl)
+        self.P = P = plan.npt
+        self.nsteps = numpy.asarray(nsteps, numpy.int64)
+        self.max_steps = int(self.nsteps.max())
+        self.kq = torch.from_numpy(numpy.ascontiguousarray(kq, numpy.float64)).to(dev)
+        self.dt = torch.from_numpy(numpy.ascontiguousarray(dt, numpy.float64)).to(dev)
+        self.ns = torch.from_numpy(numpy.ascontiguousarray(self.nsteps)).to(dev)
+        st = torch.from_numpy(numpy.ascontiguousarray(numpy.asarray(state, numpy.float64).reshape(-1, 6).T)).to(dev)
+        nck = -(-self.max_steps // self.ckpt_every)
+        self.ckpt = torch.zeros((nck + 1, 6, P), dtype=torch.float64, device=dev)
+        scratch, partial = m.workspace(plan)
+        m.track_tensors(plan, self.precision, st, self.kq, self.dt, self.ns, self.max_steps, scratch, partial,
+                        record=self.ckpt, record_every=self.ckpt_every)
+        return st.cpu().numpy().T.copy()
+
+    def backward(self, adj_final: numpy.ndarray):
+        """Reverse pass for dJ/d(final state) (P, 6).
+
+        Returns (dJ/d(initial state) (P, 6), cct_grad (ncct, 5) per global CCT:
+        [sum B-bar.dB, dJ/dc_0..3], qs_grad (Q, 2)).
+        """
+        dev = engine.device()
+        m, plan, P = self.model, self.plan, self.P
+        seg, _ = m._seg(self.precision)
+        org = m.tile_org  # FP32 tile origins: the backward kernel needs them at every precision
+        tan, tile_cct, coef = m.adjoint_data()
+        ntile = max(m.nseg_padded // N.TILE, 1)
+        nblock = max(-(-int(n) // engine.points_per_block(plan.ppl)) for n in numpy.diff(plan.design_pt_off.cpu().numpy()))
+        adj = torch.from_numpy(numpy.ascontiguousarray(numpy.asarray(adj_final, numpy.float64).reshape(-1, 6).T)).to(dev)
+        tgrad = torch.zeros((max(nblock, 1), ntile, 5), dtype=torch.float64, device=dev)
+        nq = int(m.design_qs_off_host[-1])
+        qgrad = torch.zeros((max(2 * nq, 1), P), dtype=torch.float64, device=dev)
+        state_ws = torch.empty((6, P), dtype=torch.float64, device=dev)
+        scratch, partial = m.workspace(plan)
+        tape = torch.zeros((self.ckpt_every, 36, P), dtype=torch.float64, device=dev)
+        adj_ws = torch.zeros((15, P), dtype=torch.float64, device=dev)
+        bs_pts = torch.zeros((3, P), dtype=torch.float64, device=dev)
+        bs_wts = torch.zeros((3, P), dtype=torch.float64, device=dev)
+        N.check(N.lib.cctamd_track_rk4_adjoint(
+            self.precision, plan.ppl, _p(seg), _p(org), _p(m.seg32), _p(tan), _p(tile_cct), _p(coef), ntile,
+            _p(plan.work), plan.nwork, _p(plan.wbase), _p(plan.nchunk), _p(plan.design_pt_off), m.ndesign,
+            _p(m.qs), _p(m.design_qs_off), _p(m.bconst), _p(self.ckpt), self.ckpt_every, P, _p(self.kq), _p(self.dt),
+            _p(self.ns), self.max_steps, _p(adj), _p(tgrad), _p(qgrad), _p(state_ws), _p(scratch), _p(partial),
+            _p(tape), _p(adj_ws), _p(bs_pts), _p(bs_wts), _stream()), "cctamd_track_rk4_adjoint")
+        tg = tgrad.sum(0).cpu().numpy()
+        tc = tile_cct.cpu().numpy()
+        cct_grad = numpy.zeros((m.ncct, 5))
+        for c in range(m.ncct):
+            cct_grad[c] = tg[tc == c].sum(0)
+        q = qgrad.cpu().numpy()[: 2 * nq].reshape(nq, 2, P).sum(2)
+        return adj.cpu().numpy().T.copy(), cct_grad, q
+
+
+class DesignProblem:
+    """theta -> beamline -> tracked beam, with dJ/dtheta by the adjoint.
+
+    ``builder(theta) -> Beamline`` must keep the winding counts and the orbit
+    fixed (tilts, currents and QS gradients may vary).  ``particles(beamline)``
+    gives the initial RunningParticles (independent of theta).
+    """
+
+    def __init__(self, builder: Callable, particles: Callable, length: float, footstep: float,
+                 precision: int = 33, ckpt_every: int = 50, fd_step: float = 1e-6):
+        self.builder = builder
+        self.particles = particles
+        self.length = length
+        self.footstep = footstep
+        self.precision = precision
+        self.ckpt_every = ckpt_every
+        self.fd_step = fd_step
+
+    def _setup(self, theta: numpy.ndarray):
+        bl = self.builder(theta)
+        ps = self.particles(bl)
+        state = numpy.array([p.position.to_list() + p.velocity.to_list() for p in ps])
+        kq = numpy.array([p.e / p.relativistic_mass for p in ps])
+        sd = [engine.rk4_steps(self.length, self.footstep, p.speed) for p in ps]
+        return bl, state, kq, numpy.array([d for _, d in sd]), numpy.array([s for s, _ in sd], numpy.int64)
+
+    def forward(self, theta: numpy.ndarray) -> numpy.ndarray:
+        self.theta = numpy.asarray(theta, numpy.float64).copy()
+        bl, state, kq, dt, ns = self._setup(self.theta)
+        self.tracker = AdjointTracker([bl], self.precision, self.ckpt_every)
+        self.beamline = bl
+        return self.tracker.forward(state, kq, dt, ns, [len(state)])
+
+    def _magnet_params(self, theta: numpy.ndarray):
+        """Per CCT (current, tilt angles, sinh eta) and per QS (g, s) of the design at theta."""
+        from .cct import CCT
+        from .magnets import QS
+        bl = self.builder(theta)
+        ccts, qss = [], []
+        for m in bl.magnets:
+            if isinstance(m, CCT):
+                ccts.append(m)
+            elif isinstance(m, QS):
+                qss.append(m)
+        return ccts, qss
+
+    def backward(self, adj_final: numpy.ndarray) -> numpy.ndarray:
+        """dJ/dtheta for dJ/d(final state)."""
+        _, cct_grad, qs_grad = self.tracker.backward(adj_final)
+        model = self.tracker.model
+        ccts0, qss0 = self._magnet_params(self.theta)
+        # global CCT index -> CCT object of design 0 (zero-current CCTs carry no segments)
+        gl = {k: c for c, (d, k) in enumerate(model.cct_design)}
+        grad = numpy.zeros_like(self.theta)
+        for kth in range(self.theta.size):
+            h = self.fd_step * max(1.0, abs(self.theta[kth]))
+            tp, tm = self.theta.copy(), self.theta.copy()
+            tp[kth] += h
+            tm[kth] -= h
+            cp, qp = self._magnet_params(tp)
+            cm, qm = self._magnet_params(tm)
+            g = 0.0
+            for i, c0 in enumerate(ccts0):
+                if i not in gl:
+                    continue
+                cg = cct_grad[gl[i]]
+                dI = (cp[i].current - cm[i].current) / (2 * h)
+                if dI != 0.0:
+                    g += cg[0] / c0.current * dI
+                sh = math.sinh(c0.eta)
+                for j, a0 in enumerate(c0.tilt_angles[:4]):
+                    da = (cp[i].tilt_angles[j] - cm[i].tilt_angles[j]) / (2 * h)
+                    if da == 0.0:
+                        continue
+                    r = math.radians(a0)
+                    dc_da = -(math.pi / 180.0) / math.sin(r) ** 2 / ((j + 1) * sh)
+                    g += cg[1 + j] * dc_da * da
+            for i, q0 in enumerate(qss0):
+                g += qs_grad[i, 0] * (qp[i].gradient - qm[i].gradient) / (2 * h)
+                g += qs_grad[i, 1] * (qp[i].second_gradient - qm[i].second_gradient) / (2 * h)
+            grad[kth] = g
+        return grad
+
+
+class _TrackFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta: torch.Tensor, problem: DesignProblem) -> torch.Tensor:
+        out = problem.forward(theta.detach().cpu().numpy())
+        ctx.problem = problem
+        return torch.from_numpy(out)
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        g = ctx.problem.backward(grad_out.detach().cpu().numpy())
+        return torch.from_numpy(g), None
+
+
+def track_fn(theta: torch.Tensor, problem: DesignProblem) -> torch.Tensor:
+    """Final states (P, 6) [x, y, z, vx, vy, vz] as a differentiable function of theta."""
+    return _TrackFn.apply(theta, problem)

@@ -66,7 +66,8 @@ def tune(model: "engine.DeviceModel", npt: Sequence[int], precision: int, points
             return disk
     cus = engine.cu_count()
     best: Optional[dict] = None
-    for ppl in PPL:
+    ppls = PPL + ((-2, -4) if precision == 33 else ())
+    for ppl in ppls:
         for per_cu in ITEMS_PER_CU:
             plan = model.plan(npt, ppl, per_cu * cus)
             _, partial = model.workspace(plan)

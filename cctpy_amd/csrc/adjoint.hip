@@ -1,0 +1,431 @@
+// Adjoint (reverse-mode) of the RK4 tracking path for MI355X (gfx950).
+//
+// Discrete adjoint of one RK4 step Y' = Y + h/6 (k1 + 2 k2 + 2 k3 + k4) with
+// k = [V, A], A = kq V x B(P) (final_code/packages/base_utils.py:303-309,
+// particles.py:526-535), stages reversed 4 -> 1:
+//   A_k-bar -> V_k-bar += kq (B_k x A_k-bar),  B_k-bar = kq (A_k-bar x V_k),
+//   P_k-bar = J_B(P_k)^T B_k-bar  (backward Biot-Savart kernel + QS),
+// and the parameter gradients of the field:
+//   current of a CCT:   sum over its segments of B-bar . dB_i  (B is linear in I)
+//   tilt coefficient j: sum_i dJ/dkl_i . dkl_i/dc_j + dJ/dm_i . dm_i/dc_j with
+//                       dP/dc_j = dP/dphi * sin((j+1) ksi)  (cct.py:166-188, 213-224)
+//   QS gradients:       analytic (magnets.py:637-668).
+// The forward is recomputed from checkpoints every K steps; the recompute
+// records a tape of stage points, velocities and fields.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace cctamd_detail {
+namespace {
+
+constexpr int kGradTilts = 4;   // tilt slots carried by the adjoint
+constexpr int kTanFloats = 16;  // per segment: T_i.xyz, s_i[4], 0, T_i+1.xyz, s_i+1[4], 0
+constexpr int kTileGrad = 1 + kGradTilts;
+
+// ---------------------------------------------------------- tangents
+// dP/dphi at ksi in the local frame: (-t sinh(eta) sin(phi), t sinh(eta) cos(phi), 0).
+__device__ inline void cct_tangent(const double* __restrict__ prm, int64_t i, float out[8]) {
+    const double ksi = prm[15] + prm[16] * (double)i;
+    const double phi = cct_phi(prm, ksi);
+    const double t = prm[12] / (prm[14] - cos(ksi));
+    const double lx = -t * prm[13] * sin(phi), ly = t * prm[13] * cos(phi);
+    const double* XI = prm + 3;
+    const double* YI = prm + 6;
+    for (int c = 0; c < 3; ++c) out[c] = (float)(XI[c] * lx + YI[c] * ly);
+    const int ntilt = (int)prm[20];
+    for (int j = 0; j < kGradTilts; ++j) out[3 + j] = j < ntilt ? (float)sin((double)(j + 1) * ksi) : 0.0f;
+    out[7] = 0.0f;
+}
+
+__global__ void k_winding_tangents(const double* __restrict__ params, const int64_t* __restrict__ count_off,
+                                   const int64_t* __restrict__ dst, int64_t ncct, int64_t nseg_total,
+                                   float* __restrict__ seg_tan) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nseg_total) return;
+    int64_t lo = 0, hi = ncct - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (count_off[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const double* prm = params + lo * CCTAMD_CCT_PARAMS;
+    const int64_t i = g - count_off[lo];
+    float* out = seg_tan + (dst[lo] + i) * kTanFloats;
+    cct_tangent(prm, i, out);
+    cct_tangent(prm, i + 1, out + 8);
+}
+
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// -------------------------------------------------- backward Biot-Savart
+// For each point p with weight Bb (the adjoint of the field at p):
+//   pbar(p)  = sum_i (Bb x kl_i) w_i - 3 r_i (Bb . (kl_i x r_i)) w_i / |r_i|^2,  w = |r|^-3
+// per segment i (summed over the points of the wave):
+//   dJ/dkl_i = sum_p (r x Bb) w,   dJ/dm_i = -sum_p u_p,i   (u = the pbar term)
+// contracted at once with the winding tangents into per-tile accumulators
+// tgrad[block][tile] = {sum Bb . dB (current), dJ/dc_0..3 (tilt coefficients)}.
+// FP32 terms on re-centred tiles, FP64 across tiles.
+template <int PPL>
+__global__ __launch_bounds__(kWG) void k_bs_backward_f32(
+    const float* __restrict__ seg32, const double* __restrict__ tile_org, const float* __restrict__ seg_tan,
+    const int32_t* __restrict__ tile_cct, const double* __restrict__ cct_coef, const Work* __restrict__ work,
+    const double* __restrict__ pts, const double* __restrict__ wts, int64_t ptot, int64_t ntile_total,
+    double* __restrict__ partial, double* __restrict__ tgrad) {
+    constexpr int PB = kWave * PPL;
+    const Work w = work[blockIdx.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double px[PPL], py[PPL], pz[PPL], wx[PPL], wy[PPL], wz[PPL];
+    load_points<PPL>(pts, ptot, w.pt_start, w.pt_count, lane, px, py, pz);
+    load_points<PPL>(wts, ptot, w.pt_start, w.pt_count, lane, wx, wy, wz);
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {  // surplus lanes carry no weight
+        if (lane + kWave * j >= w.pt_count) wx[j] = wy[j] = wz[j] = 0.0;
+    }
+    float bx[PPL], by[PPL], bz[PPL];
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+        bx[j] = (float)wx[j];
+        by[j] = (float)wy[j];
+        bz[j] = (float)wz[j];
+    }
+    double ax[PPL], ay[PPL], az[PPL];
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) ax[j] = ay[j] = az[j] = 0.0;
+
+    const int tend = w.tile_start + w.tile_count;
+    for (int t = w.tile_start + wave; t < tend; t += kWavesPerWG) {
+        const double ox = tile_org[4 * t], oy = tile_org[4 * t + 1], oz = tile_org[4 * t + 2];
+        const int cct = tile_cct[t];
+        const float coef = cct >= 0 ? (float)cct_coef[cct] : 0.0f;
+        float qx[PPL], qy[PPL], qz[PPL], sx[PPL], sy[PPL], sz[PPL];
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            qx[j] = (float)(px[j] - ox);
+            qy[j] = (float)(py[j] - oy);
+            qz[j] = (float)(pz[j] - oz);
+            sx[j] = sy[j] = sz[j] = 0.0f;
+        }
+        float cur = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+        const float4* s = reinterpret_cast<const float4*>(seg32) + (int64_t)t * kTile * 2;
+        const float4* tn = reinterpret_cast<const float4*>(seg_tan) + (int64_t)t * kTile * 4;
+#pragma unroll 2
+        for (int k = 0; k < kTile; ++k) {
+            const float4 kl = s[2 * k];
+            const float4 m = s[2 * k + 1];
+            float gkx = 0.0f, gky = 0.0f, gkz = 0.0f, gmx = 0.0f, gmy = 0.0f, gmz = 0.0f;
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                const float rx = qx[j] - m.x, ry = qy[j] - m.y, rz = qz[j] - m.z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float w3 = inv * inv * inv;
+                const float tx = kl.y * rz - kl.z * ry, ty = kl.z * rx - kl.x * rz, tz = kl.x * ry - kl.y * rx;
+                const float sdot = bx[j] * tx + by[j] * ty + bz[j] * tz;
+                const float a3 = 3.0f * sdot * w3 * (inv * inv);
+                const float ux = (by[j] * kl.z - bz[j] * kl.y) * w3 - rx * a3;
+                const float uy = (bz[j] * kl.x - bx[j] * kl.z) * w3 - ry * a3;
+                const float uz = (bx[j] * kl.y - by[j] * kl.x) * w3 - rz * a3;
+                sx[j] += ux;
+                sy[j] += uy;
+                sz[j] += uz;
+                gmx -= ux;
+                gmy -= uy;
+                gmz -= uz;
+                gkx += (ry * bz[j] - rz * by[j]) * w3;
+                gky += (rz * bx[j] - rx * bz[j]) * w3;
+                gkz += (rx * by[j] - ry * bx[j]) * w3;
+                cur += sdot * w3;
+            }
+            // d kl/dc_j = coef (T' s'_j - T s_j), d m/dc_j = (T' s'_j + T s_j)/2
+            const float4 t0 = tn[4 * k], t1 = tn[4 * k + 1], t2 = tn[4 * k + 2], t3 = tn[4 * k + 3];
+            const float Ax = coef * gkx + 0.5f * gmx, Ay = coef * gky + 0.5f * gmy, Az = coef * gkz + 0.5f * gmz;
+            const float Bx = 0.5f * gmx - coef * gkx, By = 0.5f * gmy - coef * gky, Bz = 0.5f * gmz - coef * gkz;
+            const float alpha = Ax * t2.x + Ay * t2.y + Az * t2.z;  // at the segment's end point
+            const float beta = Bx * t0.x + By * t0.y + Bz * t0.z;   // at its start point
+            g0 += t2.w * alpha + t0.w * beta;
+            g1 += t3.x * alpha + t1.x * beta;
+            g2 += t3.y * alpha + t1.y * beta;
+            g3 += t3.z * alpha + t1.z * beta;
+        }
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            ax[j] += (double)sx[j];
+            ay[j] += (double)sy[j];
+            az[j] += (double)sz[j];
+        }
+        cur = wave_sum(cur);
+        g0 = wave_sum(g0);
+        g1 = wave_sum(g1);
+        g2 = wave_sum(g2);
+        g3 = wave_sum(g3);
+        if (lane == 0 && cct >= 0) {
+            double* tg = tgrad + ((int64_t)w.block * ntile_total + t) * kTileGrad;
+            tg[0] += (double)cur;
+            tg[1] += (double)g0;
+            tg[2] += (double)g1;
+            tg[3] += (double)g2;
+            tg[4] += (double)g3;
+        }
+    }
+    write_partial<PPL>(ax, ay, az, wave, lane, partial + partial_base(w, PB), w.nchunk);
+}
+
+// ------------------------------------------------------ stage adjoints
+// Per-particle adjoint rows (adj[r][P]): 0-2 pbar, 3-5 vbar (end of step),
+// 6-8 sum of P_k-bar, 9-11 sum of V_k-bar, 12-14 V_k-bar of the stage just done.
+// bs_pts / bs_wts: points and weights of the next backward Biot-Savart pass.
+#pragma clang fp contract(off)
+__device__ inline void cross3(const double a[3], const double b[3], double o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// QS part of P-bar at point p with weight bb, and the QS gradients.
+__device__ inline void qs_adjoint(int64_t d, const double* __restrict__ qs, const int64_t* __restrict__ design_qs_off,
+                                  const double p[3], const double bb[3], double pbar[3], double* __restrict__ qgrad,
+                                  int64_t ptot, int64_t g) {
+    for (int64_t k = design_qs_off[d]; k < design_qs_off[d + 1]; ++k) {
+        const double* q = qs + 16 * k;
+        double x, y;
+        if (!qs_local(q, p, x, y)) continue;
+        const double gq = q[13], sq = q[14];
+        const double bxi = bb[0] * q[3] + bb[1] * q[4] + bb[2] * q[5];
+        const double byi = bb[0] * q[6] + bb[1] * q[7] + bb[2] * q[8];
+        const double dx = bxi * (sq * y) + byi * (gq + sq * x);
+        const double dy = bxi * (gq + sq * x) - byi * (sq * y);
+        for (int c = 0; c < 3; ++c) pbar[c] = pbar[c] + (q[3 + c] * dx + q[6 + c] * dy);
+        qgrad[(2 * k) * ptot + g] += bxi * y + byi * x;
+        qgrad[(2 * k + 1) * ptot + g] += bxi * (x * y) + byi * (0.5 * (x * x - y * y));
+    }
+}
+
+__device__ inline void load3(const double* __restrict__ a, int64_t ptot, int64_t g, double o[3]) {
+    o[0] = a[g];
+    o[1] = a[ptot + g];
+    o[2] = a[2 * ptot + g];
+}
+
+__device__ inline void store3(double* __restrict__ a, int64_t ptot, int64_t g, const double v[3]) {
+    a[g] = v[0];
+    a[ptot + g] = v[1];
+    a[2 * ptot + g] = v[2];
+}
+
+__device__ inline void set_bs(double* __restrict__ bs_pts, double* __restrict__ bs_wts, int64_t ptot, int64_t g,
+                              const double p[3], const double w[3], bool active) {
+    const double far[3] = {1.0e3, 1.0e3, 1.0e3}, zero[3] = {0.0, 0.0, 0.0};
+    store3(bs_pts, ptot, g, active ? p : far);
+    store3(bs_wts, ptot, g, active ? w : zero);
+}
+
+// Stage 4 of the step: A4-bar = h/6 vbar, V4-bar = h/6 pbar + kq B4 x A4-bar, B4-bar = kq A4-bar x V4.
+__global__ void k_adj_begin(const double* __restrict__ tape, double* __restrict__ adj, const double* __restrict__ kq,
+                            const double* __restrict__ dt, const int64_t* __restrict__ nsteps, int64_t step,
+                            int64_t ptot, double* __restrict__ bs_pts, double* __restrict__ bs_wts) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ptot) return;
+    const bool active = step < nsteps[g];
+    double pb[3], vb[3], P4[3], V4[3], B4[3];
+    load3(adj, ptot, g, pb);
+    load3(adj + 3 * ptot, ptot, g, vb);
+    const double* t4 = tape + 3 * 9 * ptot;
+    load3(t4, ptot, g, P4);
+    load3(t4 + 3 * ptot, ptot, g, V4);
+    load3(t4 + 6 * ptot, ptot, g, B4);
+    const double h6 = dt[g] / 6, k = kq[g];
+    double A[3], c[3], Vb[3], Bb[3];
+    for (int i = 0; i < 3; ++i) A[i] = h6 * vb[i];
+    cross3(B4, A, c);
+    for (int i = 0; i < 3; ++i) Vb[i] = h6 * pb[i] + k * c[i];
+    cross3(A, V4, c);
+    for (int i = 0; i < 3; ++i) Bb[i] = k * c[i];
+    const double zero[3] = {0.0, 0.0, 0.0};
+    store3(adj + 6 * ptot, ptot, g, zero);
+    store3(adj + 9 * ptot, ptot, g, Vb);
+    store3(adj + 12 * ptot, ptot, g, Vb);
+    set_bs(bs_pts, bs_wts, ptot, g, P4, Bb, active);
+}
+
+// After the backward pass at stage K+1 (1-based): P_{K+1}-bar, then stage K.
+//   K=3: A3 = 2h/6 vbar + h V4b,   V3b = 2h/6 pbar + h P4b + kq B3 x A3
+//   K=2: A2 = 2h/6 vbar + h/2 V3b, V2b = 2h/6 pbar + h/2 P3b + kq B2 x A2
+//   K=1: A1 = h/6 vbar + h/2 V2b,  V1b = h/6 pbar + h/2 P2b + kq B1 x A1
+//   K=0: only P1b; then pbar += sum P_k-bar, vbar += sum V_k-bar.
+template <int K>
+__global__ __launch_bounds__(kWG) void k_adj_stage(int ppl, const double* __restrict__ partial,
+                                                   const int64_t* __restrict__ design_wbase,
+                                                   const int32_t* __restrict__ design_nchunk,
+                                                   const int64_t* __restrict__ design_pt_off, int64_t nd,
+                                                   const double* __restrict__ qs,
+                                                   const int64_t* __restrict__ design_qs_off,
+                                                   const double* __restrict__ tape, double* __restrict__ adj,
+                                                   const double* __restrict__ kq, const double* __restrict__ dt,
+                                                   const int64_t* __restrict__ nsteps, int64_t step, int64_t ptot,
+                                                   double* __restrict__ bs_pts, double* __restrict__ bs_wts,
+                                                   double* __restrict__ qgrad) {
+    const int64_t g = (int64_t)blockIdx.x * kWavesPerWG + (threadIdx.x >> 6);
+    if (g >= ptot) return;
+    const bool active = step < nsteps[g];
+    double pb[3], vb[3], ps[3], vsum[3], vk[3], Pn[3], Bn[3];
+    load3(adj, ptot, g, pb);
+    load3(adj + 3 * ptot, ptot, g, vb);
+    load3(adj + 6 * ptot, ptot, g, ps);
+    load3(adj + 9 * ptot, ptot, g, vsum);
+    load3(adj + 12 * ptot, ptot, g, vk);
+    load3(bs_pts, ptot, g, Pn);  // P_{K+1} and its weight B_{K+1}-bar
+    load3(bs_wts, ptot, g, Bn);
+    const double h = dt[g], k = kq[g];
+    const PointSlot sl = point_slot(g, ppl, partial, design_wbase, design_nchunk, design_pt_off, nd);
+    if (!active) return;
+    double Pb[3];
+    chunk_sum(sl, Pb);
+    if ((threadIdx.x & (kWave - 1)) != 0) return;
+    qs_adjoint(sl.design, qs, design_qs_off, Pn, Bn, Pb, qgrad, ptot, g);
+    for (int i = 0; i < 3; ++i) ps[i] = ps[i] + Pb[i];
+    if (K == 0) {
+        for (int i = 0; i < 3; ++i) {
+            pb[i] = pb[i] + ps[i];
+            vb[i] = vb[i] + vsum[i];
+        }
+        store3(adj, ptot, g, pb);
+        store3(adj + 3 * ptot, ptot, g, vb);
+        return;
+    }
+    const double cv = K == 1 ? h / 6 : 2 * h / 6;   // weight of vbar/pbar in stage K
+    const double cp = K == 3 ? h : h / 2;           // stage K+1 point = p + cp V_K
+    const double* tk = tape + (K - 1) * 9 * ptot;
+    double P[3], V[3], B[3];
+    load3(tk, ptot, g, P);
+    load3(tk + 3 * ptot, ptot, g, V);
+    load3(tk + 6 * ptot, ptot, g, B);
+    double A[3], c[3], Vb[3], Bb[3];
+    for (int i = 0; i < 3; ++i) A[i] = cv * vb[i] + cp * vk[i];
+    cross3(B, A, c);
+    for (int i = 0; i < 3; ++i) Vb[i] = cv * pb[i] + cp * Pb[i] + k * c[i];
+    cross3(A, V, c);
+    for (int i = 0; i < 3; ++i) Bb[i] = k * c[i];
+    for (int i = 0; i < 3; ++i) vsum[i] = vsum[i] + Vb[i];
+    store3(adj + 6 * ptot, ptot, g, ps);
+    store3(adj + 9 * ptot, ptot, g, vsum);
+    store3(adj + 12 * ptot, ptot, g, Vb);
+    set_bs(bs_pts, bs_wts, ptot, g, P, Bb, true);
+}
+#pragma clang fp contract(fast)
+
+__global__ void k_copy_rows(const double* __restrict__ src, double* __restrict__ dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+template <int PPL>
+hipError_t launch_bwd(const float* seg32, const double* tile_org, const float* seg_tan, const int32_t* tile_cct,
+                      const double* cct_coef, const Work* work, int64_t nwork, const double* pts, const double* wts,
+                      int64_t ptot, int64_t ntile, double* partial, double* tgrad, hipStream_t st) {
+    if (nwork == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bs_backward_f32<PPL>, dim3((unsigned)nwork), dim3(kWG), 0, st, seg32, tile_org, seg_tan,
+                       tile_cct, cct_coef, work, pts, wts, ptot, ntile, partial, tgrad);
+    return hipGetLastError();
+}
+
+hipError_t launch_bwd_any(int ppl, const float* seg32, const double* tile_org, const float* seg_tan,
+                          const int32_t* tile_cct, const double* cct_coef, const Work* work, int64_t nwork,
+                          const double* pts, const double* wts, int64_t ptot, int64_t ntile, double* partial,
+                          double* tgrad, hipStream_t st) {
+    switch (ppl) {
+        case 1: return launch_bwd<1>(seg32, tile_org, seg_tan, tile_cct, cct_coef, work, nwork, pts, wts, ptot, ntile,
+                                     partial, tgrad, st);
+        case 2: return launch_bwd<2>(seg32, tile_org, seg_tan, tile_cct, cct_coef, work, nwork, pts, wts, ptot, ntile,
+                                     partial, tgrad, st);
+        default: return launch_bwd<4>(seg32, tile_org, seg_tan, tile_cct, cct_coef, work, nwork, pts, wts, ptot,
+                                      ntile, partial, tgrad, st);
+    }
+}
+
+}  // namespace
+}  // namespace cctamd_detail
+
+using namespace cctamd_detail;
+
+extern "C" {
+
+int cctamd_winding_tangents(const double* params, const int64_t* count_off, const int64_t* dst, int64_t ncct,
+                            int64_t nseg_total, float* seg_tan, void* stream) {
+    if (ncct < 0 || nseg_total < 0) return CCTAMD_EINVAL;
+    if (ncct == 0 || nseg_total == 0) return CCTAMD_OK;
+    if (!params || !count_off || !dst || !seg_tan) return CCTAMD_EINVAL;
+    const int64_t blocks = ceil_div(nseg_total, 256);
+    if (blocks > 0x7fffffff) return CCTAMD_ERANGE;
+    hipLaunchKernelGGL(k_winding_tangents, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, params,
+                       count_off, dst, ncct, nseg_total, seg_tan);
+    return hip_status(hipGetLastError());
+}
+
+int cctamd_track_rk4_adjoint(int precision, int ppl, const void* seg, const double* tile_org, const float* seg32,
+                             const float* seg_tan, const int32_t* tile_cct, const double* cct_coef, int64_t ntile,
+                             const int32_t* work, int64_t nwork, const int64_t* design_wbase,
+                             const int32_t* design_nchunk, const int64_t* design_pt_off, int64_t ndesign,
+                             const double* qs, const int64_t* design_qs_off, const double* bconst,
+                             const double* checkpoints, int64_t ckpt_every, int64_t nparticles, const double* kq,
+                             const double* dt, const int64_t* nsteps, int64_t max_steps, double* adj, double* tgrad,
+                             double* qgrad, double* state_ws, double* scratch, double* partial, double* tape,
+                             double* adj_ws, double* bs_pts, double* bs_wts, void* stream) {
+    if (!valid_common(precision, ppl, seg, tile_org, nwork, ndesign) || ppl < 0) return CCTAMD_EINVAL;
+    if (nparticles < 0 || max_steps < 0 || ckpt_every <= 0 || ntile < 0) return CCTAMD_EINVAL;
+    if (nparticles == 0 || max_steps == 0) return CCTAMD_OK;
+    if (!checkpoints || !kq || !dt || !nsteps || !adj || !tgrad || !qgrad || !state_ws || !scratch || !tape ||
+        !adj_ws || !bs_pts || !bs_wts || !design_wbase || !design_nchunk || !design_pt_off || !design_qs_off ||
+        !bconst)
+        return CCTAMD_EINVAL;
+    // the backward kernel reads the FP32 tiles and their origins whatever the forward precision
+    if (nwork > 0 && (!work || !partial || !seg32 || !tile_org || !seg_tan || !tile_cct || !cct_coef))
+        return CCTAMD_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t P = nparticles;
+    const DeviceSet ds{precision, ppl, seg, tile_org, (const Work*)work, nwork, design_wbase, design_nchunk,
+                       design_pt_off, ndesign, qs, design_qs_off, bconst};
+    const int64_t eblk = ceil_div(P, 256), rblk = ceil_div(P, kWavesPerWG);
+    if (rblk > 0x7fffffff) return CCTAMD_ERANGE;
+    hipError_t e;
+    // adj rows 0-5 hold the seed; copy into the working rows
+    hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)ceil_div(6 * P, 256)), dim3(256), 0, st, adj, adj_ws, 6 * P);
+    const int64_t nck = ceil_div(max_steps, ckpt_every);
+    for (int64_t c = nck - 1; c >= 0; --c) {
+        const int64_t s0 = c * ckpt_every;
+        const int64_t s1 = s0 + ckpt_every < max_steps ? s0 + ckpt_every : max_steps;
+        hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)ceil_div(6 * P, 256)), dim3(256), 0, st,
+                           checkpoints + c * 6 * P, state_ws, 6 * P);
+        e = launch_rk4_steps(ds, state_ws, P, kq, dt, nsteps, s0, s1, scratch, partial, nullptr, 1, tape, st);
+        if (e != hipSuccess) return (int)e;
+        for (int64_t step = s1 - 1; step >= s0; --step) {
+            const double* tp = tape + (step - s0) * 36 * P;
+            hipLaunchKernelGGL(k_adj_begin, dim3((unsigned)eblk), dim3(256), 0, st, tp, adj_ws, kq, dt, nsteps, step,
+                               P, bs_pts, bs_wts);
+#define CCTAMD_ADJ(K)                                                                                            \
+    e = launch_bwd_any(ppl, seg32, tile_org, seg_tan, tile_cct, cct_coef, (const Work*)work, nwork, bs_pts,      \
+                       bs_wts, P, ntile, partial, tgrad, st);                                                     \
+    if (e != hipSuccess) return (int)e;                                                                          \
+    hipLaunchKernelGGL(k_adj_stage<K>, dim3((unsigned)rblk), dim3(kWG), 0, st, ppl, partial, design_wbase,       \
+                       design_nchunk, design_pt_off, ndesign, qs, design_qs_off, tp, adj_ws, kq, dt, nsteps, step, \
+                       P, bs_pts, bs_wts, qgrad);                                                                \
+    e = hipGetLastError();                                                                                       \
+    if (e != hipSuccess) return (int)e;
+            CCTAMD_ADJ(3)
+            CCTAMD_ADJ(2)
+            CCTAMD_ADJ(1)
+            CCTAMD_ADJ(0)
+#undef CCTAMD_ADJ
+        }
+    }
+    hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)ceil_div(6 * P, 256)), dim3(256), 0, st, adj_ws, adj, 6 * P);
+    return hip_status(hipGetLastError());
+}
+
+}  // extern "C"

@@ -4,51 +4,29 @@
 // Kernel map (one HIP stream, no allocation inside the library):
 //   k_winding_generate  CCT parameters -> FP64 segments in the global frame
 //   k_pack_fp32         FP64 segments  -> FP32 tiles re-centred on a per-tile
-//                                         FP64 origin
-//   k_bs_partial<P,N>   split-K Biot-Savart: one workgroup = one point block x
+//                                         FP64 origin (+ K/C rows)
+//   k_bs_partial_*<N>   split-K Biot-Savart: one workgroup = one point block x
 //                       one chunk of tiles; the 4 waves share the block and
 //                       interleave tiles; segments are wave-uniform (scalar
 //                       loads into SGPRs), points live in VGPRs (N per lane)
 //   k_field_reduce      fixed-order sum of the chunk partials + QS + uniform
 //   k_rk4_stage<S>      same reduction, then RK4 stage S of each particle
 //
-// The C ABI is declared in include/cctamd.h.
+// The adjoint kernels are in adjoint.hip; the C ABI is declared in
+// include/cctamd.h.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include <algorithm>
-#include <vector>
 
-#include "../../include/cctamd.h"
+#include "common.hpp"
 
+namespace cctamd_detail {
 namespace {
 
-constexpr int kTile = CCTAMD_TILE;
-constexpr int kWave = 64;
-constexpr int kWavesPerWG = 4;
-constexpr int kWG = kWave * kWavesPerWG;
-
 // ------------------------------------------------------------ geometry
-// Mirrors CCT.__init__/phi_ksi_function/BipolarToroidalCoordinateSystem.convert
-// (final_code/packages/cct.py:96-164, 166-188, 213-224) with the same
-// operation order; contraction is off so each product is rounded like Python's.
 #pragma clang fp contract(off)
-__device__ inline void cct_point(const double* __restrict__ prm, int64_t i, double out[3]) {
-    const double ksi = prm[15] + prm[16] * (double)i;
-    double phi = prm[17] * ksi + prm[18];
-    const int ntilt = (int)prm[20];
-    const int skip = (int)prm[29];
-    for (int j = 0; j < ntilt && j < CCTAMD_MAX_TILTS; ++j) {
-        if ((skip >> j) & 1) continue;
-        phi = phi + prm[21 + j] * sin((double)(j + 1) * ksi);
-    }
-    const double t = prm[12] / (prm[14] - cos(ksi));
-    out[0] = t * prm[13] * cos(phi);
-    out[1] = t * prm[13] * sin(phi);
-    out[2] = t * sin(ksi);
-}
-
 __global__ void k_winding_generate(const double* __restrict__ params, const int64_t* __restrict__ count_off,
                                    const int64_t* __restrict__ dst, int64_t ncct, int64_t nseg_total,
                                    double* __restrict__ seg64) {
@@ -88,7 +66,6 @@ __global__ void k_winding_points(const double* __restrict__ params, int64_t npoi
     out[3 * i + 1] = p[1];
     out[3 * i + 2] = p[2];
 }
-#pragma clang fp contract(fast)
 
 __global__ void k_pack_fp32(const double* __restrict__ seg64, int64_t nseg, float* __restrict__ seg32,
                             double* __restrict__ tile_org, float* __restrict__ seg32k) {
@@ -129,55 +106,9 @@ __global__ void k_pack_fp32(const double* __restrict__ seg64, int64_t nseg, floa
     }
 }
 
+#pragma clang fp contract(fast)
+
 // ----------------------------------------------------- Biot-Savart core
-// One work item: a point block of one design x one chunk of its tiles.
-// Partials of design d live at design_wbase[d]*3*PB and are laid out
-// [block][point][component][chunk], so that the reduction over chunks reads
-// contiguous memory.
-struct Work {
-    int32_t design, pt_start, pt_count, tile_start, tile_count, chunk, nchunk, block;
-};
-
-__device__ inline int64_t partial_base(const Work& w, int pb) {
-    const int64_t wbase = (int64_t)blockIdx.x - (int64_t)w.block * w.nchunk - w.chunk;
-    return wbase * 3 * pb + (int64_t)w.block * pb * 3 * w.nchunk + w.chunk;
-}
-
-template <int PPL>
-__device__ inline void load_points(const double* __restrict__ pts, int64_t ptot, int pt_start, int pt_count,
-                                   int lane, double px[PPL], double py[PPL], double pz[PPL]) {
-#pragma unroll
-    for (int j = 0; j < PPL; ++j) {
-        int idx = lane + kWave * j;
-        idx = idx < pt_count ? idx : 0;  // surplus lanes duplicate point 0, results discarded
-        const int64_t g = (int64_t)pt_start + idx;
-        px[j] = pts[g];
-        py[j] = pts[ptot + g];
-        pz[j] = pts[2 * ptot + g];
-    }
-}
-
-// The 4 waves of the workgroup hold partial sums for the same point block;
-// they are added in wave order (deterministic) and written as one partial
-// per (point, component): element (i, c) at out[(i*3 + c) * nchunk].
-template <int PPL>
-__device__ inline void write_partial(double ax[PPL], double ay[PPL], double az[PPL], int wave, int lane,
-                                     double* __restrict__ out, int nchunk) {
-    constexpr int PB = kWave * PPL;
-    __shared__ double red[kWavesPerWG][3][PB];
-#pragma unroll
-    for (int j = 0; j < PPL; ++j) {
-        red[wave][0][lane + kWave * j] = ax[j];
-        red[wave][1][lane + kWave * j] = ay[j];
-        red[wave][2][lane + kWave * j] = az[j];
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < 3 * PB; k += kWG) {
-        const int c = k / PB, i = k % PB;
-        out[(int64_t)(i * 3 + c) * nchunk] = ((red[0][c][i] + red[1][c][i]) + red[2][c][i]) + red[3][c][i];
-    }
-}
-
 // FP32 terms on tiles re-centred at an FP64 origin; each tile's 32 terms are
 // summed in FP32 and the tile sums are accumulated in FP64.
 template <int PPL>
@@ -294,6 +225,91 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k(const float* __restrict
     write_partial<PPL>(ax, ay, az, wave, lane, partial + partial_base(w, PB), w.nchunk);
 }
 
+// Small point blocks (32 or 16 points): each wave is split into G lane groups
+// that hold the same points and walk different tiles, so every lane works.
+// The workgroup stages 4G tiles of K/C rows in LDS per round; a lane reads
+// the segments of its group's tile with LDS broadcasts (2 or 4 addresses per
+// wave), which issue outside the VALU.  Missing tiles of the last round are
+// zero-current segments far away.
+template <int G>
+__global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __restrict__ seg32k,
+                                                              const double* __restrict__ tile_org,
+                                                              const Work* __restrict__ work,
+                                                              const double* __restrict__ pts, int64_t ptot,
+                                                              double* __restrict__ partial) {
+    constexpr int PB = kWave / G;
+    constexpr int NS = kWavesPerWG * G;   // tile streams per workgroup
+    constexpr int ROW = kTile * 3;        // float4 per tile
+    __shared__ float4 lds[NS][ROW + 1];   // +1: the streams start in different banks
+    __shared__ double red[kWavesPerWG][3][PB];
+    const Work w = work[blockIdx.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int grp = lane / PB;
+    const int stream = wave * G + grp;
+    int idx = lane % PB;
+    idx = idx < w.pt_count ? idx : 0;
+    const int64_t gp = (int64_t)w.pt_start + idx;
+    const double px = pts[gp], py = pts[ptot + gp], pz = pts[2 * ptot + gp];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    const int tend = w.tile_start + w.tile_count;
+    const float4* src = reinterpret_cast<const float4*>(seg32k);
+    const float4 pad0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 pad1 = make_float4(0.0f, 0.0f, 1.0e3f, 1.0e3f);
+    const float4 pad2 = make_float4(1.0e3f, 0.0f, 0.0f, 0.0f);
+    for (int t0 = w.tile_start; t0 < tend; t0 += NS) {
+        for (int i = threadIdx.x; i < NS * ROW; i += kWG) {
+            const int s = i / ROW, e = i % ROW;
+            const int t = t0 + s;
+            lds[s][e] = t < tend ? src[(int64_t)t * ROW + e] : (e % 3 == 0 ? pad0 : (e % 3 == 1 ? pad1 : pad2));
+        }
+        __syncthreads();
+        const int t = t0 + stream < tend ? t0 + stream : w.tile_start;
+        const float qx = (float)(px - tile_org[4 * t]);
+        const float qy = (float)(py - tile_org[4 * t + 1]);
+        const float qz = (float)(pz - tile_org[4 * t + 2]);
+        float kx = 0.0f, ky = 0.0f, kz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        const float4* r4 = lds[stream];
+#pragma unroll 8
+        for (int k = 0; k < kTile; ++k) {
+            const float4 a = r4[3 * k];
+            const float4 b = r4[3 * k + 1];
+            const float4 c = r4[3 * k + 2];
+            const float rx = qx - b.z, ry = qy - b.w, rz = qz - c.x;
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float w3 = inv * inv * inv;
+            kx += a.x * w3;
+            ky += a.y * w3;
+            kz += a.z * w3;
+            cx += a.w * w3;
+            cy += b.x * w3;
+            cz += b.y * w3;
+        }
+        ax += (double)((ky * qz - kz * qy) - cx);
+        ay += (double)((kz * qx - kx * qz) - cy);
+        az += (double)((kx * qy - ky * qx) - cz);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int off = PB; off < kWave; off <<= 1) {  // add the lane groups (fixed order)
+        ax += __shfl_xor(ax, off);
+        ay += __shfl_xor(ay, off);
+        az += __shfl_xor(az, off);
+    }
+    if (grp == 0) {
+        red[wave][0][lane] = ax;
+        red[wave][1][lane] = ay;
+        red[wave][2][lane] = az;
+    }
+    __syncthreads();
+    double* out = partial + partial_base(w, PB);
+    for (int k = threadIdx.x; k < 3 * PB; k += kWG) {
+        const int c = k / PB, i = k % PB;
+        out[(int64_t)(i * 3 + c) * w.nchunk] = ((red[0][c][i] + red[1][c][i]) + red[2][c][i]) + red[3][c][i];
+    }
+}
+
 // Plain FP64 terms on global-frame segments.
 template <int PPL>
 __global__ __launch_bounds__(kWG) void k_bs_partial_f64(const double* __restrict__ seg64,
@@ -332,83 +348,6 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f64(const double* __restrict
 }
 
 // ------------------------------------------------ reduction + QS + RK4
-#pragma clang fp contract(off)
-// Hard-edge QS field (final_code/packages/magnets.py:637-668), same comparisons.
-__device__ inline void qs_add(const double* __restrict__ q, const double p[3], double b[3]) {
-    const double d0 = p[0] + (-q[0]), d1 = p[1] + (-q[1]), d2 = p[2] + (-q[2]);
-    const double x = q[3] * d0 + q[4] * d1 + q[5] * d2;
-    const double y = q[6] * d0 + q[7] * d1 + q[8] * d2;
-    const double z = q[9] * d0 + q[10] * d1 + q[11] * d2;
-    const double L = q[12], g = q[13], s = q[14], a = q[15];
-    if (z < 0 || z > L) return;
-    if (fabs(x) > a || fabs(y) > a || sqrt(x * x + y * y) > a) return;
-    const double bx = g * y + s * (x * y);
-    const double by = g * x + 0.5 * s * (x * x - y * y);
-    b[0] = b[0] + (q[3] * bx + q[6] * by);
-    b[1] = b[1] + (q[4] * bx + q[7] * by);
-    b[2] = b[2] + (q[5] * bx + q[8] * by);
-}
-#pragma clang fp contract(fast)
-
-__device__ inline int64_t find_design(const int64_t* __restrict__ off, int64_t nd, int64_t g) {
-    int64_t lo = 0, hi = nd - 1;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= g) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
-// Location of point g's partials: design, then the [component][chunk] rows.
-struct PointSlot {
-    int64_t design;
-    const double* rows;  // rows[c * nc + chunk] for component c
-    int nc;
-};
-
-__device__ inline PointSlot point_slot(int64_t g, int ppl, const double* __restrict__ partial,
-                                       const int64_t* __restrict__ design_wbase,
-                                       const int32_t* __restrict__ design_nchunk,
-                                       const int64_t* __restrict__ design_pt_off, int64_t nd) {
-    PointSlot s;
-    s.design = find_design(design_pt_off, nd, g);
-    const int pb = kWave * ppl;
-    const int64_t lp = g - design_pt_off[s.design];
-    s.nc = design_nchunk[s.design];
-    s.rows = partial + design_wbase[s.design] * 3 * pb + ((lp / pb) * pb + lp % pb) * 3 * (int64_t)s.nc;
-    return s;
-}
-
-// Fixed-order sum over chunks (lanes stride the chunks, then a butterfly):
-// deterministic, and every lane ends with the total.
-__device__ inline void chunk_sum(const PointSlot& s, double b[3]) {
-    const int lane = threadIdx.x & (kWave - 1);
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int c = lane; c < s.nc; c += kWave) {
-        s0 += s.rows[c];
-        s1 += s.rows[s.nc + c];
-        s2 += s.rows[2 * s.nc + c];
-    }
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        s0 += __shfl_xor(s0, off);
-        s1 += __shfl_xor(s1, off);
-        s2 += __shfl_xor(s2, off);
-    }
-    b[0] = s0;
-    b[1] = s1;
-    b[2] = s2;
-}
-
-__device__ inline void analytic_fields(int64_t d, const double* __restrict__ qs,
-                                       const int64_t* __restrict__ design_qs_off,
-                                       const double* __restrict__ bconst, const double p[3], double b[3]) {
-    for (int64_t k = design_qs_off[d]; k < design_qs_off[d + 1]; ++k) qs_add(qs + 16 * k, p, b);
-    b[0] = b[0] + bconst[4 * d];
-    b[1] = b[1] + bconst[4 * d + 1];
-    b[2] = b[2] + bconst[4 * d + 2];
-}
-
 __global__ __launch_bounds__(kWG) void k_field_reduce(int ppl, const double* __restrict__ partial,
                                                       const int64_t* __restrict__ design_wbase,
                                                       const int32_t* __restrict__ design_nchunk,
@@ -458,7 +397,8 @@ __global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __rest
                                                    double* __restrict__ scratch, int64_t ptot,
                                                    const double* __restrict__ kq, const double* __restrict__ dt,
                                                    const int64_t* __restrict__ nsteps, int64_t step,
-                                                   double* __restrict__ record) {
+                                                   double* __restrict__ record, int64_t record_every,
+                                                   double* __restrict__ tape) {
     const int64_t g = (int64_t)blockIdx.x * kWavesPerWG + (threadIdx.x >> 6);
     if (g >= ptot) return;
     double* ps = scratch;
@@ -487,9 +427,18 @@ __global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __rest
     chunk_sum(sl, b);
     if ((threadIdx.x & (kWave - 1)) != 0) return;
     analytic_fields(sl.design, qs, design_qs_off, bconst, p, b);
+    if (tape) {  // stage point, velocity and field for the adjoint
+        double* t = tape + S * 9 * ptot;
+        for (int c = 0; c < 3; ++c) {
+            t[c * ptot + g] = p[c];
+            t[(3 + c) * ptot + g] = v[c];
+            t[(6 + c) * ptot + g] = b[c];
+        }
+    }
     const double cr[3] = {v[1] * b[2] - v[2] * b[1], -v[0] * b[2] + v[2] * b[0], v[0] * b[1] - v[1] * b[0]};
     const double a[3] = {cr[0] * k, cr[1] * k, cr[2] * k};
     const double h2 = h / 2;
+    const bool rec = record && ((step + 1) % record_every == 0);
     for (int c = 0; c < 3; ++c) {
         const int64_t o = c * ptot + g;
         if (S == 0) {
@@ -515,8 +464,8 @@ __global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __rest
             state[3 * ptot + o] = nv;
             ps[o] = np;
             vs[o] = nv;
-            if (record) {
-                double* r = record + (step + 1) * 6 * ptot;
+            if (rec) {
+                double* r = record + ((step + 1) / record_every) * 6 * ptot;
                 r[o] = np;
                 r[3 * ptot + o] = nv;
             }
@@ -524,11 +473,6 @@ __global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __rest
     }
 }
 #pragma clang fp contract(fast)
-
-// ------------------------------------------------------------ launchers
-inline int hip_status(hipError_t e) { return e == hipSuccess ? CCTAMD_OK : (int)e; }
-
-inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 template <int PPL>
 hipError_t launch_bs(int precision, const void* seg, const double* tile_org, const Work* work, int64_t nwork,
@@ -546,8 +490,32 @@ hipError_t launch_bs(int precision, const void* seg, const double* tile_org, con
     return hipGetLastError();
 }
 
+}  // namespace
+
+// ------------------------------------------------------- shared launchers
+bool valid_common(int precision, int ppl, const void* seg, const double* tile_org, int64_t nwork,
+                  int64_t ndesign) {
+    if (precision != 32 && precision != 33 && precision != 64) return false;
+    if (!valid_ppl(ppl) || (ppl < 0 && precision != 33)) return false;
+    if (nwork < 0 || ndesign <= 0 || nwork > 0x7fffffff) return false;
+    if (nwork > 0 && seg == nullptr) return false;
+    if (nwork > 0 && precision != 64 && tile_org == nullptr) return false;
+    return true;
+}
+
 hipError_t launch_bs_any(int precision, int ppl, const void* seg, const double* tile_org, const Work* work,
                          int64_t nwork, const double* pts, int64_t ptot, double* partial, hipStream_t st) {
+    if (ppl < 0) {
+        if (nwork == 0) return hipSuccess;
+        if (precision != 33) return hipErrorInvalidValue;
+        if (ppl == -2)
+            hipLaunchKernelGGL(k_bs_partial_f32k_grp<2>, dim3((unsigned)nwork), dim3(kWG), 0, st,
+                               (const float*)seg, tile_org, work, pts, ptot, partial);
+        else
+            hipLaunchKernelGGL(k_bs_partial_f32k_grp<4>, dim3((unsigned)nwork), dim3(kWG), 0, st,
+                               (const float*)seg, tile_org, work, pts, ptot, partial);
+        return hipGetLastError();
+    }
     switch (ppl) {
         case 1: return launch_bs<1>(precision, seg, tile_org, work, nwork, pts, ptot, partial, st);
         case 2: return launch_bs<2>(precision, seg, tile_org, work, nwork, pts, ptot, partial, st);
@@ -555,20 +523,39 @@ hipError_t launch_bs_any(int precision, int ppl, const void* seg, const double* 
     }
 }
 
-bool valid_common(int precision, int ppl, const void* seg, const double* tile_org, int64_t nwork,
-                  int64_t ndesign) {
-    if (precision != 32 && precision != 33 && precision != 64) return false;
-    if (ppl != 1 && ppl != 2 && ppl != 4) return false;
-    if (nwork < 0 || ndesign <= 0 || nwork > 0x7fffffff) return false;
-    if (nwork > 0 && seg == nullptr) return false;
-    if (nwork > 0 && precision != 64 && tile_org == nullptr) return false;
-    return true;
+hipError_t launch_rk4_steps(const DeviceSet& ds, double* state, int64_t ptot, const double* kq, const double* dt,
+                            const int64_t* nsteps, int64_t step0, int64_t step1, double* scratch, double* partial,
+                            double* record, int64_t record_every, double* tape, hipStream_t st) {
+    const int64_t eblocks = ceil_div(ptot, 256);
+    const int64_t rblocks = ceil_div(ptot, kWavesPerWG);
+    hipLaunchKernelGGL(k_rk4_init, dim3((unsigned)eblocks), dim3(256), 0, st, state, ptot, scratch,
+                       step0 == 0 ? record : nullptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (int64_t step = step0; step < step1; ++step) {
+        double* tp = tape ? tape + (step - step0) * 36 * ptot : nullptr;
+#define CCTAMD_STAGE(S)                                                                                          \
+    e = launch_bs_any(ds.precision, ds.ppl, ds.seg, ds.tile_org, ds.work, ds.nwork, scratch, ptot, partial, st); \
+    if (e != hipSuccess) return e;                                                                               \
+    hipLaunchKernelGGL(k_rk4_stage<S>, dim3((unsigned)rblocks), dim3(kWG), 0, st, ds.ppl, partial,               \
+                       ds.design_wbase, ds.design_nchunk, ds.design_pt_off, ds.ndesign, ds.qs, ds.design_qs_off,  \
+                       ds.bconst, state, scratch, ptot, kq, dt, nsteps, step, record, record_every, tp);          \
+    e = hipGetLastError();                                                                                       \
+    if (e != hipSuccess) return e;
+        CCTAMD_STAGE(0)
+        CCTAMD_STAGE(1)
+        CCTAMD_STAGE(2)
+        CCTAMD_STAGE(3)
+#undef CCTAMD_STAGE
+    }
+    return hipSuccess;
 }
 
-}  // namespace
+}  // namespace cctamd_detail
+
+using namespace cctamd_detail;
 
 extern "C" {
-
 int cctamd_version(void) { return 100; }
 
 const char* cctamd_error_string(int code) {
@@ -620,9 +607,9 @@ int64_t cctamd_plan(int64_t ndesign, const int64_t* design_ntile, const int64_t*
                     const int64_t* design_tile_off, const int64_t* design_pt_off, int ppl, int64_t target_items,
                     int64_t min_tiles, int32_t* work, int64_t* design_wbase, int32_t* design_nchunk) {
     if (ndesign <= 0 || !design_ntile || !design_npt) return CCTAMD_EINVAL;
-    if (ppl != 1 && ppl != 2 && ppl != 4) return CCTAMD_EINVAL;
+    if (!valid_ppl(ppl)) return CCTAMD_EINVAL;
     if (target_items <= 0 || min_tiles <= 0) return CCTAMD_EINVAL;
-    const int64_t pb = kWave * ppl;
+    const int64_t pb = pb_of(ppl);
     int64_t total = 0;
     for (int64_t d = 0; d < ndesign; ++d) {
         if (design_ntile[d] < 0 || design_npt[d] < 0) return CCTAMD_EINVAL;
@@ -699,38 +686,19 @@ int cctamd_track_rk4(int precision, int ppl, const void* seg, const double* tile
                      const int64_t* design_pt_off, int64_t ndesign, const double* qs, const int64_t* design_qs_off,
                      const double* bconst, double* state, int64_t nparticles, const double* kq, const double* dt,
                      const int64_t* nsteps, int64_t max_steps, double* scratch, double* partial, double* record,
-                     void* stream) {
+                     int64_t record_every, void* stream) {
     if (!valid_common(precision, ppl, seg, tile_org, nwork, ndesign)) return CCTAMD_EINVAL;
-    if (nparticles < 0 || max_steps < 0) return CCTAMD_EINVAL;
+    if (nparticles < 0 || max_steps < 0 || record_every <= 0) return CCTAMD_EINVAL;
     if (nparticles == 0) return CCTAMD_OK;
     if (!state || !kq || !dt || !nsteps || !scratch || !design_wbase || !design_nchunk || !design_pt_off ||
         !design_qs_off || !bconst)
         return CCTAMD_EINVAL;
     if (nwork > 0 && (!work || !partial)) return CCTAMD_EINVAL;
-    hipStream_t st = (hipStream_t)stream;
-    const int64_t eblocks = ceil_div(nparticles, 256);
-    const int64_t rblocks = ceil_div(nparticles, kWavesPerWG);
-    if (rblocks > 0x7fffffff) return CCTAMD_ERANGE;
-    hipLaunchKernelGGL(k_rk4_init, dim3((unsigned)eblocks), dim3(256), 0, st, state, nparticles, scratch, record);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return (int)e;
-    const Work* wk = (const Work*)work;
-    for (int64_t step = 0; step < max_steps; ++step) {
-#define CCTAMD_STAGE(S)                                                                                          \
-    e = launch_bs_any(precision, ppl, seg, tile_org, wk, nwork, scratch, nparticles, partial, st);               \
-    if (e != hipSuccess) return (int)e;                                                                          \
-    hipLaunchKernelGGL(k_rk4_stage<S>, dim3((unsigned)rblocks), dim3(kWG), 0, st, ppl, partial, design_wbase,   \
-                       design_nchunk, design_pt_off, ndesign, qs, design_qs_off, bconst, state, scratch,         \
-                       nparticles, kq, dt, nsteps, step, record);                                                \
-    e = hipGetLastError();                                                                                       \
-    if (e != hipSuccess) return (int)e;
-        CCTAMD_STAGE(0)
-        CCTAMD_STAGE(1)
-        CCTAMD_STAGE(2)
-        CCTAMD_STAGE(3)
-#undef CCTAMD_STAGE
-    }
-    return CCTAMD_OK;
+    if (ceil_div(nparticles, kWavesPerWG) > 0x7fffffff) return CCTAMD_ERANGE;
+    const DeviceSet ds{precision, ppl, seg, tile_org, (const Work*)work, nwork, design_wbase, design_nchunk,
+                       design_pt_off, ndesign, qs, design_qs_off, bconst};
+    return hip_status(launch_rk4_steps(ds, state, nparticles, kq, dt, nsteps, 0, max_steps, scratch, partial, record,
+                                       record_every, nullptr, (hipStream_t)stream));
 }
 
 }  // extern "C"

@@ -50,6 +50,11 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def points_per_block(ppl: int) -> int:
+    """Point-block size of a points-per-lane code (1, 2, 4 -> 64 ppl; -2, -4 -> 32, 16)."""
+    return 64 * ppl if ppl > 0 else 64 // (-ppl)
+
+
 def cu_count() -> int:
     return int(torch.cuda.get_device_properties(device()).multi_processor_count)
 
@@ -91,13 +96,21 @@ class Plan:
 class DeviceModel:
     """D designs resident in HBM (FP64 segments, FP32 tiles, QS records, uniform fields)."""
 
-    def __init__(self, specs: Sequence[DesignSpec], fp32: bool = True):
+    def __init__(self, specs: Sequence[DesignSpec], fp32: bool = True, align_cct: bool = False):
+        """``align_cct`` starts every CCT on a tile boundary (needed by the adjoint,
+        which attributes each tile to one CCT)."""
         dev = device()
         self.specs = list(specs)
         self.ndesign = D = len(self.specs)
         if D == 0:
             raise ValueError("no design given")
-        nseg = numpy.array([s.nseg for s in self.specs], numpy.int64)
+        self.align_cct = align_cct
+        if align_cct:
+            nseg = numpy.array([-(-int(s.raw_kl.shape[0]) // N.TILE) * N.TILE
+                                + sum(-(-int(c) // N.TILE) * N.TILE for c in s.cct_counts)
+                                for s in self.specs], numpy.int64)
+        else:
+            nseg = numpy.array([s.nseg for s in self.specs], numpy.int64)
         self.design_ntile = (nseg + N.TILE - 1) // N.TILE
         self.design_tile_off = numpy.concatenate([[0], numpy.cumsum(self.design_ntile)]).astype(numpy.int64)
         self.design_nseg = nseg
@@ -107,7 +120,7 @@ class DeviceModel:
         # pad segments: kl = 0 and a midpoint far from every field point
         seg64 = torch.zeros((max(S, 1), 8), dtype=torch.float64, device=dev)
         seg64[:, 4:7] = 1.0e3
-        params, counts, dst = [], [], []
+        params, counts, dst, cct_design = [], [], [], []
         for d, s in enumerate(self.specs):
             row = int(self.design_tile_off[d]) * N.TILE
             nraw = int(s.raw_kl.shape[0])
@@ -121,15 +134,24 @@ class DeviceModel:
                 c = int(s.cct_counts[k])
                 if c == 0:
                     continue
+                if align_cct:
+                    row = -(-row // N.TILE) * N.TILE
                 params.append(s.cct_params[k])
                 counts.append(c)
                 dst.append(row)
+                cct_design.append((d, k))
                 row += c
         self.ncct = len(params)
+        self.cct_design = cct_design          # global CCT index -> (design, CCT index within the design)
+        self.cct_dst = numpy.array(dst, numpy.int64)
+        self.cct_counts = numpy.array(counts, numpy.int64)
+        self.cct_params = numpy.array(params, numpy.float64).reshape(len(params), N.CCT_PARAMS)
+        self._gen = None
         if self.ncct:
-            prm = torch.from_numpy(numpy.ascontiguousarray(numpy.array(params, numpy.float64))).to(dev)
+            prm = torch.from_numpy(numpy.ascontiguousarray(self.cct_params)).to(dev)
             off = torch.from_numpy(numpy.concatenate([[0], numpy.cumsum(counts)]).astype(numpy.int64)).to(dev)
-            dsts = torch.from_numpy(numpy.array(dst, numpy.int64)).to(dev)
+            dsts = torch.from_numpy(self.cct_dst).to(dev)
+            self._gen = (prm, off, dsts)
             N.check(N.lib.cctamd_winding_generate(_p(prm), _p(off), _p(dsts), self.ncct, int(sum(counts)),
                                                   _p(seg64), _stream()), "cctamd_winding_generate")
         self.seg64 = seg64
@@ -151,6 +173,30 @@ class DeviceModel:
             b[d, :3] = s.bconst
         self.bconst = torch.from_numpy(b).to(dev)
         self._plans: Dict[tuple, Plan] = {}
+
+    # --------------------------------------------------------- adjoint data
+    def adjoint_data(self):
+        """(seg_tan [S][16] f32, tile_cct [T] i32, cct_coef [ncct] f64) for cctamd_track_rk4_adjoint."""
+        if not self.align_cct:
+            raise ValueError("the adjoint needs a model built with align_cct=True")
+        if getattr(self, "_adj", None) is None:
+            dev = device()
+            S = self.nseg_padded
+            tan = torch.zeros((max(S, 1), 16), dtype=torch.float32, device=dev)
+            if self.ncct:
+                prm, off, dsts = self._gen
+                N.check(N.lib.cctamd_winding_tangents(_p(prm), _p(off), _p(dsts), self.ncct,
+                                                      int(self.cct_counts.sum()), _p(tan), _stream()),
+                        "cctamd_winding_tangents")
+            nt = max(S // N.TILE, 1)
+            tc = numpy.full(nt, -1, numpy.int32)
+            for c in range(self.ncct):
+                t0 = int(self.cct_dst[c]) // N.TILE
+                t1 = -(-(int(self.cct_dst[c]) + int(self.cct_counts[c])) // N.TILE)
+                tc[t0:t1] = c
+            coef = self.cct_params[:, 19].copy() if self.ncct else numpy.zeros(1)
+            self._adj = (tan, torch.from_numpy(tc).to(dev), torch.from_numpy(numpy.ascontiguousarray(coef)).to(dev))
+        return self._adj
 
     # ------------------------------------------------------------- plans
     def plan(self, design_npt: Sequence[int], ppl: int = 2, target_items: Optional[int] = None) -> "Plan":
@@ -204,7 +250,7 @@ class DeviceModel:
         dev = device()
         soa = torch.from_numpy(numpy.ascontiguousarray(pts.T)).to(dev)
         out = torch.empty_like(soa)
-        partial = torch.empty(max(plan.nwork * 3 * 64 * ppl, 1), dtype=torch.float64, device=dev)
+        partial = torch.empty(max(plan.nwork * 3 * points_per_block(ppl), 1), dtype=torch.float64, device=dev)
         seg, org = self._seg(precision)
         N.check(N.lib.cctamd_bs_field(precision, ppl, _p(seg), _p(org), _p(plan.work), plan.nwork, _p(plan.wbase),
                                       _p(plan.nchunk), _p(plan.design_pt_off), self.ndesign, _p(self.qs),
@@ -236,13 +282,13 @@ class DeviceModel:
         dt_t = torch.from_numpy(numpy.ascontiguousarray(dt, numpy.float64)).to(dev)
         ns_t = torch.from_numpy(numpy.ascontiguousarray(ns)).to(dev)
         scratch = torch.empty((12, plan.npt), dtype=torch.float64, device=dev)
-        partial = torch.empty(max(plan.nwork * 3 * 64 * ppl, 1), dtype=torch.float64, device=dev)
+        partial = torch.empty(max(plan.nwork * 3 * points_per_block(ppl), 1), dtype=torch.float64, device=dev)
         rec = torch.empty((max_steps + 1, 6, plan.npt), dtype=torch.float64, device=dev) if record else None
         seg, org = self._seg(precision)
         N.check(N.lib.cctamd_track_rk4(precision, ppl, _p(seg), _p(org), _p(plan.work), plan.nwork, _p(plan.wbase),
                                        _p(plan.nchunk), _p(plan.design_pt_off), self.ndesign, _p(self.qs),
                                        _p(self.design_qs_off), _p(self.bconst), _p(soa), plan.npt, _p(kq_t),
-                                       _p(dt_t), _p(ns_t), max_steps, _p(scratch), _p(partial), _p(rec),
+                                       _p(dt_t), _p(ns_t), max_steps, _p(scratch), _p(partial), _p(rec), 1,
                                        _stream()), "cctamd_track_rk4")
         out = soa.cpu().numpy().T.copy()
         return out, (rec.cpu().numpy().transpose(0, 2, 1).copy() if rec is not None else None)
@@ -252,18 +298,18 @@ class DeviceModel:
         """(scratch [12][P], partial) device buffers for a plan."""
         dev = device()
         return (torch.empty((12, max(plan.npt, 1)), dtype=torch.float64, device=dev),
-                torch.empty(max(plan.nwork * 3 * 64 * plan.ppl, 1), dtype=torch.float64, device=dev))
+                torch.empty(max(plan.nwork * 3 * points_per_block(plan.ppl), 1), dtype=torch.float64, device=dev))
 
     def track_tensors(self, plan: "Plan", precision: int, state: torch.Tensor, kq: torch.Tensor, dt: torch.Tensor,
                       nsteps: torch.Tensor, max_steps: int, scratch: torch.Tensor, partial: torch.Tensor,
-                      record: Optional[torch.Tensor] = None) -> None:
+                      record: Optional[torch.Tensor] = None, record_every: int = 1) -> None:
         """In-place RK4 tracking of device-resident SoA state (6, P); nothing crosses PCIe."""
         seg, org = self._seg(precision)
         N.check(N.lib.cctamd_track_rk4(precision, plan.ppl, _p(seg), _p(org), _p(plan.work), plan.nwork,
                                        _p(plan.wbase), _p(plan.nchunk), _p(plan.design_pt_off), self.ndesign,
                                        _p(self.qs), _p(self.design_qs_off), _p(self.bconst), _p(state), plan.npt,
                                        _p(kq), _p(dt), _p(nsteps), int(max_steps), _p(scratch), _p(partial),
-                                       _p(record), _stream()), "cctamd_track_rk4")
+                                       _p(record), int(record_every), _stream()), "cctamd_track_rk4")
 
     def field_tensors(self, plan: "Plan", precision: int, points: torch.Tensor, partial: torch.Tensor,
                       out: torch.Tensor) -> None:
@@ -289,18 +335,19 @@ _CACHE: "collections.OrderedDict[bytes, DeviceModel]" = collections.OrderedDict(
 _CACHE_SIZE = 8
 
 
-def model_for(specs: Sequence[DesignSpec], fp32: bool = True) -> DeviceModel:
+def model_for(specs: Sequence[DesignSpec], fp32: bool = True, align_cct: bool = False) -> DeviceModel:
     """DeviceModel for the given designs, reused while the magnet parameters are unchanged."""
     h = hashlib.sha1()
     for s in specs:
         h.update(s.key())
     h.update(b"fp32" if fp32 else b"fp64")
+    h.update(b"aligned" if align_cct else b"packed")
     k = h.digest()
     m = _CACHE.get(k)
     if m is not None:
         _CACHE.move_to_end(k)
         return m
-    m = DeviceModel(specs, fp32=fp32)
+    m = DeviceModel(specs, fp32=fp32, align_cct=align_cct)
     _CACHE[k] = m
     while len(_CACHE) > _CACHE_SIZE:
         _CACHE.popitem(last=False)

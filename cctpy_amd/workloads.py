@@ -204,3 +204,49 @@ def beam_particles(beam: Beam, bl) -> list:
     return [ParticleFactory.create_from_phase_space_particle(ip, cs, PhaseSpaceParticle(c[0], c[1], c[2], c[3], 0.0,
                                                                                         c[4]))
             for c in beam.coords]
+
+
+# ---------------------------------------------------- design gradients
+THETA_NAMES = ["qs1_gradient", "qs2_gradient", "dipole_tilt_2", "dipole_tilt_3", "dipole_tilt_4",
+               "ag_tilt_1", "ag_tilt_3", "ag_tilt_4", "dipole_current", "ag_current"]
+
+
+def section_theta0(seed: int = SEEDS["agcct_section"]) -> numpy.ndarray:
+    """The 10-parameter design vector of the agcct_section workload (2 QS gradients, 3 free
+    tilts of the dipole pair, 3 of the AG pair, 2 currents)."""
+    ops = agcct_section(n_particles=1, seed=seed).designs[0]
+    ag = ops[3][1]
+    qs1, qs2 = ops[1][1], ops[5][1]
+    dt, at = ag["tilt_angles"]
+    return numpy.array([qs1["gradient"], qs2["gradient"], dt[1], dt[2], dt[3], at[0], at[2], at[3],
+                        ag["currents"][0], ag["currents"][1]], numpy.float64)
+
+
+def section_ops(theta: Sequence[float], disperse_number_per_winding: int = 720, drift_after: float = 1.1,
+                seed: int = SEEDS["agcct_section"]) -> List[Op]:
+    """agcct_section builder operations at design vector theta, plus a drift of ``drift_after``."""
+    ops = [(n, dict(kw)) for n, kw in agcct_section(n_particles=1, seed=seed,
+                                                    disperse_number_per_winding=disperse_number_per_winding).designs[0]]
+    t = [float(v) for v in theta]
+    ops[1][1]["gradient"] = t[0]
+    ops[5][1]["gradient"] = t[1]
+    ag = ops[3][1]
+    ag["tilt_angles"] = [[30.0, t[2], t[3], t[4]], [t[5], 30.0, t[6], t[7]]]
+    ag["currents"] = [t[8], t[9]]
+    if drift_after:
+        ops.append(("append_drift", dict(length=drift_after)))
+    return ops
+
+
+def beam_spot_objective(final: "torch.Tensor", ideal_position, frame_x, frame_y, delta) -> "torch.Tensor":
+    """rms x + rms y + |D| at the end plane, D = least-squares slope of x against delta; x, y are
+    offsets from the ideal particle along the ideal particle's frame axes."""
+    import torch
+    pos = final[:, :3] - torch.as_tensor(ideal_position, dtype=final.dtype)
+    x = pos @ torch.as_tensor(frame_x, dtype=final.dtype)
+    y = pos @ torch.as_tensor(frame_y, dtype=final.dtype)
+    d = torch.as_tensor(delta, dtype=final.dtype)
+    dc = d - d.mean()
+    xc = x - x.mean()
+    slope = (xc * dc).sum() / (dc * dc).sum()
+    return torch.sqrt((xc * xc).mean()) + torch.sqrt(((y - y.mean()) ** 2).mean()) + slope.abs()

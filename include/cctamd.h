@@ -79,7 +79,9 @@ int cctamd_pack_fp32(const double *seg64, int64_t nseg, float *seg32, double *ti
 /*
  * Host-only planner: splits (design, point block, tile chunk) work.
  *   design_ntile[d], design_npt[d], design_tile_off[d], design_pt_off[d] (host).
- *   ppl: points per lane (1, 2 or 4); a point block holds 64*ppl points.
+ *   ppl: points per lane (1, 2 or 4): a point block holds 64*ppl points; or -2 / -4:
+ *        blocks of 32 / 16 points, each wave split into 2 / 4 lane groups walking
+ *        different tiles (precision 33 only).
  * With work == NULL returns the number of work items only. Otherwise fills
  * work[nwork][8] = {design, pt_start, pt_count, tile_start, tile_count, chunk, nchunk, block},
  * design_wbase[d] (first item of design d) and design_nchunk[d].
@@ -100,8 +102,8 @@ int64_t cctamd_plan(int64_t ndesign, const int64_t *design_ntile, const int64_t 
  *              re-centred tile, FP64 accumulation across tiles) or 33 (seg32k +
  *              tile_org: the same with per-tile sums K = sum w kl, C = sum w kl x m and
  *              B_tile = K x q - C; for points away from the windings, e.g. in a bore).
- *   partial: device workspace of nwork*3*64*ppl doubles; design d's partials start at
- *            design_wbase[d]*3*64*ppl, laid out [block][point][component][chunk].
+ *   partial: device workspace of nwork*3*PB doubles (PB = points per block); design d's
+ *            partials start at design_wbase[d]*3*PB, laid out [block][point][component][chunk].
  */
 int cctamd_bs_field(int precision, int ppl, const void *seg, const double *tile_org,
                     const int32_t *work, int64_t nwork, const int64_t *design_wbase,
@@ -129,7 +131,9 @@ int cctamd_bs_partial(int precision, int ppl, const void *seg, const double *til
  *   scratch  double[12][P] device workspace.
  *   kq[P] = e/m, dt[P], nsteps[P] (int64) per particle; steps beyond nsteps[i]
  *   leave particle i unchanged; max_steps = max(nsteps).
- *   record   nullable double[max_steps+1][6][P]: state after every step.
+ *   record   nullable double[max_steps/record_every+1][6][P]: the state at step 0 and
+ *            after every record_every-th step (row k = state after k*record_every
+ *            steps; rows past a particle's own step count are left untouched).
  */
 int cctamd_track_rk4(int precision, int ppl, const void *seg, const double *tile_org,
                      const int32_t *work, int64_t nwork, const int64_t *design_wbase,
@@ -137,7 +141,46 @@ int cctamd_track_rk4(int precision, int ppl, const void *seg, const double *tile
                      const double *qs, const int64_t *design_qs_off, const double *bconst,
                      double *state, int64_t nparticles, const double *kq, const double *dt,
                      const int64_t *nsteps, int64_t max_steps, double *scratch, double *partial,
-                     double *record, void *stream);
+                     double *record, int64_t record_every, void *stream);
+
+/*
+ * Winding tangents for the adjoint: per segment 16 floats
+ * {dP/dphi(ksi_i).xyz, sin((j+1) ksi_i) j=0..3, 0, dP/dphi(ksi_i+1).xyz, sin((j+1) ksi_i+1), 0}
+ * in the global frame (dP/dc_j = dP/dphi * sin((j+1) ksi), c_j the tilt coefficient of
+ * final_code/packages/cct.py:179-187).  Arguments as for cctamd_winding_generate.
+ */
+int cctamd_winding_tangents(const double *params, const int64_t *count_off, const int64_t *dst,
+                            int64_t ncct, int64_t nseg_total, float *seg_tan, void *stream);
+
+/*
+ * Reverse-mode derivative of cctamd_track_rk4 (no counterpart in the reference,
+ * which has no gradients; it differentiates the tracking that
+ * final_code/packages/particles.py:561-632 performs).
+ *   precision/seg: the forward path to recompute (32, 33 or 64);
+ *   seg32, tile_org, seg_tan, tile_cct[ntile], cct_coef[ncct] (= 1e-7 I): FP32
+ *     re-centred tiles of a segment set whose CCTs each start on a tile boundary;
+ *     the backward kernel uses them at every precision, so tile_org (the FP32 tile
+ *     origins of cctamd_pack_fp32) is required also when precision is 64;
+ *   checkpoints: double[ceil(max_steps/ckpt_every)][6][P], the forward state every
+ *     ckpt_every steps (cctamd_track_rk4 with record_every = ckpt_every);
+ *   adj: double[6][P], in: dJ/d(final x, v); out: dJ/d(initial x, v);
+ *   tgrad: double[nblock][ntile][5] (+=): per tile sum B-bar.dB (current; divide by
+ *     I) and dJ/dc_0..3 (tilt coefficients);
+ *   qgrad: double[2*Q][P] (+=): dJ/d(gradient), dJ/d(second gradient) of each QS;
+ *   workspaces: state_ws[6][P], scratch[12][P], partial (as cctamd_bs_field),
+ *     tape[ckpt_every][36][P], adj_ws[15][P], bs_pts[3][P], bs_wts[3][P].
+ */
+int cctamd_track_rk4_adjoint(int precision, int ppl, const void *seg, const double *tile_org,
+                             const float *seg32, const float *seg_tan, const int32_t *tile_cct,
+                             const double *cct_coef, int64_t ntile, const int32_t *work, int64_t nwork,
+                             const int64_t *design_wbase, const int32_t *design_nchunk,
+                             const int64_t *design_pt_off, int64_t ndesign, const double *qs,
+                             const int64_t *design_qs_off, const double *bconst,
+                             const double *checkpoints, int64_t ckpt_every, int64_t nparticles,
+                             const double *kq, const double *dt, const int64_t *nsteps, int64_t max_steps,
+                             double *adj, double *tgrad, double *qgrad, double *state_ws, double *scratch,
+                             double *partial, double *tape, double *adj_ws, double *bs_pts, double *bs_wts,
+                             void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,82 @@
+"""GPU adjoint: gradients of tracked states and of the beam-spot objective against central
+finite differences of the FP64 forward path."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from tests._helpers import ensure_built, gpu_available
+
+pytestmark = pytest.mark.gpu
+
+ensure_built()
+
+from cctpy_amd import adjoint, workloads as W  # noqa: E402
+from cctpy_amd.cctpy import ParticleFactory  # noqa: E402
+
+if not gpu_available():
+    pytest.skip("no GPU visible", allow_module_level=True)
+
+
+def _problem(n_particles, n, length, step, precision, ckpt):
+    beam = W.gaussian_beam(np.random.default_rng(17), n_particles, 0.02)
+
+    def particles(bl):
+        return W.beam_particles(W.Beam(beam, 0.4, 250.0), bl)
+
+    prob = adjoint.DesignProblem(lambda th: W.build_beamline(W.section_ops(th, n, 0.0)), particles, length, step,
+                                 precision=precision, ckpt_every=ckpt)
+    return prob, beam
+
+
+def _fd(prob, theta, J, rel_h=1e-6):
+    g = np.zeros_like(theta)
+    for k in range(theta.size):
+        h = rel_h * max(abs(theta[k]), 1.0)
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        g[k] = (J(prob.forward(tp)) - J(prob.forward(tm))) / (2 * h)
+    return g
+
+
+def test_adjoint_linear_functional_vs_fd():
+    """J = w . final states (a linear functional): the adjoint's dJ/dtheta against central FD."""
+    prob, _ = _problem(24, 90, 1.2, 5e-3, 64, 16)
+    theta = W.section_theta0()
+    final = prob.forward(theta)
+    rng = np.random.default_rng(3)
+    w = rng.normal(size=final.shape) * np.array([1.0, 1.0, 1.0, 1e-8, 1e-8, 1e-8])
+    grad = prob.backward(w)
+    fd = _fd(prob, theta, lambda f: float(np.sum(w * f)))
+    scale = np.abs(fd).max()
+    assert np.max(np.abs(grad - fd)) / scale < 1e-4, (grad, fd)
+
+
+def test_adjoint_beam_spot_objective_torch():
+    """Config-4 style objective through torch autograd, 1.1 m drift, checkpoints every 50 steps."""
+    beam = W.gaussian_beam(np.random.default_rng(5), 32, 0.08)
+    n = 90
+
+    def particles(bl):
+        return W.beam_particles(W.Beam(beam, 0.0, 250.0), bl)
+
+    bl0 = W.build_beamline(W.section_ops(W.section_theta0(), n))
+    L = bl0.get_length()
+    ip = ParticleFactory.create_proton_along(bl0, L, 250.0)
+    fr = ip.get_natural_coordinate_system()
+    prob = adjoint.DesignProblem(lambda th: W.build_beamline(W.section_ops(th, n)), particles, L, 1e-2,
+                                 precision=64, ckpt_every=50)
+
+    def J_np(final):
+        return float(W.beam_spot_objective(torch.from_numpy(final), ip.position.to_list(), fr.XI.to_list(),
+                                           fr.YI.to_list(), beam[:, 4]))
+
+    theta = torch.tensor(W.section_theta0(), requires_grad=True)
+    final = adjoint.track_fn(theta, prob)
+    J = W.beam_spot_objective(final, ip.position.to_list(), fr.XI.to_list(), fr.YI.to_list(), beam[:, 4])
+    J.backward()
+    fd = _fd(prob, theta.detach().numpy().copy(), J_np)
+    g = theta.grad.numpy()
+    assert np.max(np.abs(g - fd)) / np.abs(fd).max() < 1e-4, (g, fd)

@@ -110,6 +110,37 @@ def test_field_fp32_kc_in_bore_accuracy():
         assert err.max() < 1e-6, (prec, err.max())
 
 
+@pytest.mark.parametrize("ppl", [-2, -4])
+def test_field_fp32_small_blocks_batched(ppl):
+    """Split-wave kernel (32- or 16-point blocks) on two designs with uneven point and segment counts."""
+    from cctpy_amd import workloads as W
+    from oracle import workload_builder as WB
+    wl = W.population(n_designs=2, n_particles=8, seed=3)
+    rng = np.random.default_rng(6)
+    pts_all, want_all, npt = [], [], []
+    for d, n in zip(wl.designs, (45, 13)):
+        ref = WB.build_beamline(d)
+        pts = []
+        for s in rng.uniform(0.3, ref.length - 0.3, n):
+            p, t = ref.traj.point_at(float(s)), ref.traj.direct_at(float(s))
+            dx, dz = rng.uniform(-0.02, 0.02), rng.uniform(-0.02, 0.02)
+            pts.append((p[0] - t[1] * dx, p[1] + t[0] * dx, dz))
+        pts = np.array(pts)
+        pts_all.append(pts)
+        want_all.append(C.field(O.beamline_arrays(ref.magnets), pts))
+        npt.append(n)
+    model = engine.model_for([device_parts(W.build_beamline(d)).spec() for d in wl.designs])
+    assert model.design_nseg[0] != model.design_nseg[1]
+    got = model.field(np.concatenate(pts_all), npt, precision=33, ppl=ppl)
+    want = np.concatenate(want_all)
+    err = np.linalg.norm(got - want, axis=1) / np.linalg.norm(want, axis=1)
+    assert err.max() < 1e-6, err.max()
+    ref2 = model.field(np.concatenate(pts_all), npt, precision=33, ppl=2)
+    assert np.max(np.linalg.norm(got - ref2, axis=1) / np.linalg.norm(want, axis=1)) < 1e-6
+    with pytest.raises(Exception):
+        model.field(np.concatenate(pts_all), npt, precision=64, ppl=ppl)
+
+
 def test_qs_fields_bitwise(golden):
     for key in ("qs_a13", "qs_c05"):
         g = golden[key]

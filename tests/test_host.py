@@ -43,12 +43,12 @@ def _plan(ntile, npt, ppl, target, min_tiles=4):
     return work[:n], wbase, nch, toff, poff
 
 
-@pytest.mark.parametrize("ppl", [1, 2, 4])
+@pytest.mark.parametrize("ppl", [1, 2, 4, -2, -4])
 def test_plan_covers_every_point_tile_pair_once(ppl):
     ntile = [37, 0, 5, 1200, 3]
     npt = [300, 17, 0, 64, 1]
     work, wbase, nch, toff, poff = _plan(ntile, npt, ppl, target=200)
-    pb = 64 * ppl
+    pb = 64 * ppl if ppl > 0 else 64 // -ppl
     cover = {}
     for d in range(len(ntile)):
         nb = -(-npt[d] // pb)
@@ -74,6 +74,18 @@ def test_plan_rejects_bad_arguments():
     assert N.lib.cctamd_plan(0, p(z), p(z), p(z), p(z), 2, 10, 4, None, None, None) == -22
     neg = np.array([-1], np.int64)
     assert N.lib.cctamd_plan(1, p(neg), p(z), p(z), p(z), 2, 10, 4, None, None, None) == -22
+    # split-wave blocks exist only for the K x q - C FP32 kernel
+    assert N.lib.cctamd_bs_partial(64, -2, p(z), p(z), p(z), 1, p(z), 1, p(z), None) == -22
+
+
+def test_adjoint_requires_fp32_tile_origins():
+    """The backward kernel reads the FP32 tile origins at every forward precision: a NULL
+    tile_org is rejected on the host (before any launch) also for precision 64."""
+    buf = np.zeros(64)
+    q = buf.ctypes.data_as(ctypes.c_void_p)
+    args = [64, 2, q, None, q, q, q, q, 1, q, 1, q, q, q, 1, q, q, q, q, 4, 1, q, q, q, 1,
+            q, q, q, q, q, q, q, q, q, q, None]
+    assert N.lib.cctamd_track_rk4_adjoint(*args) == -22
 
 
 def _frame_eq(lcs: LocalCoordinateSystem, fr: O.Frame):
