@@ -12,6 +12,11 @@ Inputs are resident in HBM before timing starts; every pass restarts from the
 same initial state with a device-to-device copy.
 
 Prints one JSON line on stdout (diagnostics go to stderr).
+
+Profiling helpers (not bench lines): ``--roofline-only`` launches only the
+Biot-Savart kernel on the workload's initial points (for counter passes),
+``--copy-mib M`` runs a plain elementwise copy of M MiB (the calibration of
+the HBM byte counters).
 """
 from __future__ import annotations
 
@@ -28,11 +33,42 @@ if ROOT not in sys.path:
 
 FLOP_PER_PAIR = 27           # r (3) + |r|^2 (5) + sqrt (1) + r^3 (2) + kl x r (9) + 1/x (1) + scale (3) + sum (3)
 PEAK_TFLOPS = {32: 157.3, 64: 78.6}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bs_kernel_traffic.json")
 METRIC = "particle-steps/s and B-S pair-evals/s vs plain FP64 GPU, at ≤1 µm position error"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_name(ppl: int) -> str:
+    return f"k_bs_partial_f32k<{ppl}>" if ppl > 0 else f"k_bs_partial_f32k_grp<{-ppl}>"
+
+
+def measured_traffic(workload: str, kernel: str, nwork: int):
+    """HBM bytes per launch of this kernel shape from the committed counter summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rows = json.load(f)["launches"]
+    except (OSError, ValueError, KeyError):
+        return None
+    for r in rows:
+        if r["workload"] == workload and r["kernel"] == kernel and r["work_items"] == nwork:
+            return r["hbm_bytes_per_launch"]
+    return None
+
+
+def copy_calibration(mib: int, reps: int = 5) -> int:
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel": "torch.neg float32", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "dispatches": reps}), flush=True)
+    return 0
 
 
 def cpu_baseline(wl, n_particles: int, n_steps: int):
@@ -54,6 +90,10 @@ def cpu_baseline(wl, n_particles: int, n_steps: int):
     t0 = time.perf_counter()
     c_oracle.track(mags, st, kq, dts, ns, threads=threads)
     el = time.perf_counter() - t0
+    n1 = max(1, n_particles // threads)  # one core: a proportional slice of the same sample
+    t0 = time.perf_counter()
+    c_oracle.track(mags, st[:n1], kq[:n1], dts[:n1], ns[:n1], threads=1)
+    el1 = time.perf_counter() - t0
     nseg = int(mags["kl"].shape[0])
     psteps = len(st) * n_steps
     model = open("/proc/cpuinfo").read().split("model name")[1].split("\n")[0].strip(" :\t") \
@@ -62,7 +102,7 @@ def cpu_baseline(wl, n_particles: int, n_steps: int):
             "sample": f"{len(st)} protons x {n_steps} RK4 steps over all {nseg} segments + QS of {wl.name} "
                       f"(C FP64 restatement of the reference CPU path, OpenMP {threads} threads, {el:.1f} s, "
                       f"host cores visible {cores}, CPU {model})",
-            "pair_evals_per_s": psteps * 4 * nseg / el}
+            "pair_evals_per_s": psteps * 4 * nseg / el, "value_1core": n1 * n_steps / el1}
 
 
 def main() -> int:
@@ -71,15 +111,20 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3, help="timed tracking passes")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="agcct_section")
-    ap.add_argument("--ppl", type=int, default=0, help="points per lane (0: autotune)")
+    ap.add_argument("--ppl", type=int, default=0, help="points per lane, -2/-4 split waves (0: autotune)")
+    ap.add_argument("--items-per-cu", type=int, default=0, help="work items per CU with --ppl (0: default)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison pass")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-sample", default="64x24", help="CPU baseline sample: particles x RK4 steps")
     ap.add_argument("--roofline-launches", type=int, default=20)
+    ap.add_argument("--roofline-only", action="store_true", help="only launch the Biot-Savart kernel (profiling)")
+    ap.add_argument("--copy-mib", type=int, default=0, help="plain copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2
+    if args.copy_mib:
+        return copy_calibration(args.copy_mib)
 
     import numpy as np
     import torch
@@ -119,11 +164,34 @@ def main() -> int:
     FP32 = 33  # FP32 tracking kernel (K x q - C tile sums)
     tuned = {}
     for prec in (FP32, 64):
-        if prec == 64 and args.no_fp64:
+        if prec == 64 and (args.no_fp64 or args.roofline_only):
             continue
-        tuned[prec] = (autotune.tune(model, npt, prec, s0[:3].contiguous()) if args.ppl == 0
-                       else {"ppl": args.ppl, "target_items": None})
+        if args.ppl == 0:
+            tuned[prec] = autotune.tune(model, npt, prec, s0[:3].contiguous())
+        else:
+            ppl = args.ppl if (prec == FP32 or args.ppl > 0) else 2
+            tuned[prec] = {"ppl": ppl, "target_items": args.items_per_cu * engine.cu_count() or None}
         log(f"[bench] fp{prec} shape: {tuned[prec]}")
+
+    def bs_launch_timing(plan, pts, partial, launches: int) -> float:
+        """Average duration (ms) of the FP32 Biot-Savart kernel, HIP events on its stream."""
+        model.bs_partial_tensors(plan, FP32, pts, partial)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(launches):
+            model.bs_partial_tensors(plan, FP32, pts, partial)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / launches
+
+    if args.roofline_only:
+        plan = model.plan(npt, tuned[FP32]["ppl"], tuned[FP32]["target_items"])
+        _, partial = model.workspace(plan)
+        avg = bs_launch_timing(plan, s0[:3].contiguous(), partial, args.roofline_launches)
+        print(json.dumps({"workload": wl.name, "kernel": kernel_name(plan.ppl), "work_items": plan.nwork,
+                          "launches": args.roofline_launches + 1, "avg_launch_ms": avg}), flush=True)
+        return 0
 
     def run(prec: int, passes: int, warmup: int):
         plan = model.plan(npt, tuned[prec]["ppl"], tuned[prec]["target_items"])
@@ -164,21 +232,15 @@ def main() -> int:
 
     # live timing of the dominant kernel (split-K Biot-Savart, FP32) on the stream it runs on
     pts = torch.from_numpy(np.ascontiguousarray(out32[:, :3].T)).to(dev)
-    model.bs_partial_tensors(plan32, FP32, pts, partial32)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(args.roofline_launches):
-        model.bs_partial_tensors(plan32, FP32, pts, partial32)
-    e1.record()
-    torch.cuda.synchronize()
-    avg_ms = e0.elapsed_time(e1) / args.roofline_launches
+    avg_ms = bs_launch_timing(plan32, pts, partial32, args.roofline_launches)
     flops = float(np.sum(np.asarray(npt) * model.design_nseg)) * FLOP_PER_PAIR
     achieved = flops / (avg_ms / 1e3) / 1e12
+    kname = kernel_name(plan32.ppl)
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[32], "unit": "TFLOP/s",
-                "frac": achieved / PEAK_TFLOPS[32], "traffic": None,
-                "kernel": f"k_bs_partial_f32k<{plan32.ppl}>", "avg_launch_ms": avg_ms,
-                "flop_per_launch": flops}
+                "frac": achieved / PEAK_TFLOPS[32], "traffic": measured_traffic(wl.name, kname, plan32.nwork),
+                "kernel": kname, "avg_launch_ms": avg_ms, "flop_per_launch": flops,
+                "algorithmic_bytes_per_launch": float(model.nseg_padded * 48 + plan32.npt * 24
+                                                      + plan32.nwork * 3 * engine.points_per_block(plan32.ppl) * 8)}
     log(f"[bench] bs kernel: {avg_ms * 1e3:.1f} us/launch, {achieved:.2f} TFLOP/s "
         f"({100 * roofline['frac']:.1f}% of FP32 vector peak)")
 
@@ -199,11 +261,12 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 Biot-Savart terms (fp64 tile accumulation) / fp64 RK4 state",
+        "dtype": "f32 Biot-Savart terms; f64 tile sums and RK4 state",
         "data": "synthetic",
         "config": {"workload": wl.name, "designs": len(bls), "segments": int(model.nseg), "protons": P,
                    "rk4_steps": max_steps, "footstep_mm": wl.footstep * 1e3, "length_m": wl.length,
-                   "parallelism": "single GPU", "ppl": plan32.ppl, "work_items": plan32.nwork},
+                   "parallelism": "single GPU", "ppl": plan32.ppl, "work_items": plan32.nwork,
+                   "target_items": tuned[FP32]["target_items"]},
         "pair_evals_per_s": pairs_per_pass * args.steps / (ms32 / 1e3),
         "fp64_plain": fp64,
         "speedup_vs_fp64": (rate32 / fp64["particle_steps_per_s"]) if fp64 else None,
