@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Throughput of RK4 proton tracking through a CCT gantry section on one MI355X.
+"""Throughput of RK4 proton tracking through CCT gantry sections on one MI355X.
 
 A *step* is one full tracking pass of the whole synthetic beam through the
 workload's beamline (all RK4 steps; default workload ``agcct_section``:
@@ -10,6 +10,13 @@ tiles, FP64 particle state); one pass of the plain FP64 path is timed beside
 it and is the reference for the final-position error.
 Inputs are resident in HBM before timing starts; every pass restarts from the
 same initial state with a device-to-device copy.
+
+Besides the main line, ``extras`` (skip with ``--no-extras``) reports
+  * the batched design population (64 designs of uneven segment counts, 32
+    protons each): FP32 forward, plain FP64 forward, and FP32 forward+adjoint
+    (beam-spot objective per design, checkpoints every 50 steps);
+  * the differentiation workload (AG-CCT section + 1.1 m drift, 256 protons):
+    FP32 forward+adjoint.
 
 Prints one JSON line on stdout (diagnostics go to stderr).
 
@@ -22,7 +29,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -33,8 +39,10 @@ if ROOT not in sys.path:
 
 FLOP_PER_PAIR = 27           # r (3) + |r|^2 (5) + sqrt (1) + r^3 (2) + kl x r (9) + 1/x (1) + scale (3) + sum (3)
 PEAK_TFLOPS = {32: 157.3, 64: 78.6}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bs_kernel_traffic.json")
 METRIC = "particle-steps/s and B-S pair-evals/s vs plain FP64 GPU, at ≤1 µm position error"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bs_kernel_traffic.json")
+FP32 = 33                    # FP32 tracking kernel (K x q - C tile sums)
+CKPT_EVERY = 50
 
 
 def log(*a):
@@ -105,6 +113,120 @@ def cpu_baseline(wl, n_particles: int, n_steps: int):
             "pair_evals_per_s": psteps * 4 * nseg / el, "value_1core": n1 * n_steps / el1}
 
 
+class Setup:
+    """A workload resident on the device: designs, particles, step counts."""
+
+    def __init__(self, wl, align_cct: bool = False):
+        import numpy as np
+        import torch
+        from cctpy_amd import engine
+        from cctpy_amd import workloads as W
+        from cctpy_amd.magnets import device_parts
+        self.wl = wl
+        self.bls = [W.build_beamline(ops) for ops in wl.designs]
+        ps, npt = [], []
+        for bl, beam in zip(self.bls, wl.beams):
+            q = W.beam_particles(beam, bl)
+            ps += q
+            npt.append(len(q))
+        self.ps, self.npt, self.P = ps, npt, len(ps)
+        state = np.array([p.position.to_list() + p.velocity.to_list() for p in ps])
+        kq = np.array([p.e / p.relativistic_mass for p in ps])
+        lengths = [wl.beam_length(i) for i, n in enumerate(npt) for _ in range(n)]
+        steps_dt = [engine.rk4_steps(L, wl.footstep, p.speed) for L, p in zip(lengths, ps)]
+        self.ns = np.array([s for s, _ in steps_dt], np.int64)
+        dt = np.array([d for _, d in steps_dt])
+        self.max_steps = int(self.ns.max())
+        t0 = time.perf_counter()
+        self.model = engine.DeviceModel([device_parts(bl).spec() for bl in self.bls], fp32=True,
+                                        align_cct=align_cct)
+        torch.cuda.synchronize()
+        self.gen_s = time.perf_counter() - t0
+        dev = engine.device()
+        self.s0 = torch.from_numpy(np.ascontiguousarray(state.T)).to(dev)
+        self.kq = torch.from_numpy(kq).to(dev)
+        self.dt = torch.from_numpy(dt).to(dev)
+        self.ns_t = torch.from_numpy(self.ns).to(dev)
+        self.st = torch.empty_like(self.s0)
+        self.particle_steps = float(self.ns.sum())
+        seg_of = np.repeat(self.model.design_nseg, npt)
+        self.pairs_per_pass = float(np.sum(self.ns * 4 * seg_of))
+        log(f"[bench] {wl.name}: {len(self.bls)} design(s), {self.model.nseg} segments, {self.P} protons, "
+            f"{self.max_steps} RK4 steps; windings generated on the GPU in {self.gen_s:.2f} s")
+
+    def shape(self, prec: int, ppl: int, items_per_cu: int) -> dict:
+        from cctpy_amd import autotune, engine
+        if ppl == 0:
+            return autotune.tune(self.model, self.npt, prec, self.s0[:3].contiguous())
+        if prec != FP32 and ppl < 0:
+            ppl = 2
+        return {"ppl": ppl, "target_items": items_per_cu * engine.cu_count() or None}
+
+    def timed(self, one_pass, passes: int, warmup: int) -> float:
+        import torch
+        for _ in range(warmup):
+            self.st.copy_(self.s0)
+            one_pass()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(passes):
+            self.st.copy_(self.s0)
+            one_pass()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    def forward(self, prec: int, shape: dict, passes: int, warmup: int, graph: bool):
+        """(ms per pass, final state (P, 6), plan, partial) of the tracking path at one precision."""
+        from cctpy_amd import engine
+        m = self.model
+        plan = m.plan(self.npt, shape["ppl"], shape["target_items"])
+        scratch, partial = m.workspace(plan)
+        if graph:
+            g = engine.TrackGraph(m, plan, prec, self.st, self.kq, self.dt, self.ns_t, self.max_steps, scratch,
+                                  partial)
+            one_pass = g.replay
+        else:
+            def one_pass():
+                m.track_tensors(plan, prec, self.st, self.kq, self.dt, self.ns_t, self.max_steps, scratch, partial)
+        ms = self.timed(one_pass, passes, warmup) / passes
+        return ms, self.st.cpu().numpy().T.copy(), plan, partial
+
+    def forward_adjoint(self, shape: dict, passes: int, warmup: int):
+        """ms per FP32 forward (with checkpoints) + objective + adjoint pass, and the split."""
+        import torch
+        from cctpy_amd.adjoint import AdjointTracker
+        from cctpy_amd.beam import BeamGroups
+        from cctpy_amd.particles import ParticleFactory
+        ppl = shape["ppl"] if shape["ppl"] > 0 else 1
+        tr = AdjointTracker(self.bls, precision=FP32, ckpt_every=CKPT_EVERY, ppl=ppl)
+        tr.prepare(self.npt, self.kq, self.dt, self.ns_t, self.max_steps)
+        ideals, frames = [], []
+        for i, (bl, beam) in enumerate(zip(self.bls, self.wl.beams)):
+            ip = ParticleFactory.create_proton_along(bl, beam.s0 + self.wl.beam_length(i), beam.kinetic_mev)
+            ideals.append(ip)
+            frames.append(ip.get_natural_coordinate_system())
+        bg = BeamGroups(ideals, frames, self.npt, [p.speed for p in self.ps],
+                        [p.compute_scalar_momentum() for p in self.ps])
+        out = {}
+
+        def one_pass():
+            tr.forward_tensors(self.st)
+            pps, stats = bg.project(self.st)
+            adj = bg.seed(pps, stats)
+            tr.backward_tensors(adj)
+            out["stats"] = stats
+
+        ms = self.timed(one_pass, passes, warmup) / passes
+        fwd = self.timed(lambda: tr.forward_tensors(self.st), 1, 0)
+        J = BeamGroups.objective(out["stats"])
+        return {"ms_per_pass": ms, "forward_ms": fwd, "adjoint_ms": ms - fwd,
+                "particle_steps_per_s": self.particle_steps / (ms / 1e3), "ppl": tr.plan.ppl,
+                "checkpoint_every": CKPT_EVERY, "objective_mean": float(J.mean()),
+                "grad_finite": bool(torch.isfinite(tr.tgrad).all().item())}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,10 +237,12 @@ def main() -> int:
     ap.add_argument("--items-per-cu", type=int, default=0, help="work items per CU with --ppl (0: default)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison pass")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extras", action="store_true", help="skip the population and adjoint extras")
     ap.add_argument("--cpu-sample", default="64x24", help="CPU baseline sample: particles x RK4 steps")
     ap.add_argument("--roofline-launches", type=int, default=20)
     ap.add_argument("--roofline-only", action="store_true", help="only launch the Biot-Savart kernel (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0, help="plain copy of this many MiB (counter calibration)")
+    ap.add_argument("--graph", action="store_true", help="replay each tracking pass as one captured HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -129,49 +253,13 @@ def main() -> int:
     import numpy as np
     import torch
 
-    from cctpy_amd import autotune, engine
+    from cctpy_amd import engine
     from cctpy_amd import workloads as W
-    from cctpy_amd.magnets import device_parts
 
-    dev = engine.device()
-    wl = W.by_name(args.workload)
-    bls = [W.build_beamline(ops) for ops in wl.designs]
-    ps, npt = [], []
-    for bl, beam in zip(bls, wl.beams):
-        q = W.beam_particles(beam, bl)
-        ps += q
-        npt.append(len(q))
-    P = len(ps)
-    state = np.array([p.position.to_list() + p.velocity.to_list() for p in ps])
-    kq = np.array([p.e / p.relativistic_mass for p in ps])
-    lengths = [wl.beam_length(i) for i, n in enumerate(npt) for _ in range(n)]
-    steps_dt = [engine.rk4_steps(L, wl.footstep, p.speed) for L, p in zip(lengths, ps)]
-    ns = np.array([s for s, _ in steps_dt], np.int64)
-    dt = np.array([d for _, d in steps_dt])
-    max_steps = int(ns.max())
-    t0 = time.perf_counter()
-    model = engine.DeviceModel([device_parts(bl).spec() for bl in bls], fp32=True)
-    torch.cuda.synchronize()
-    log(f"[bench] {wl.name}: {len(bls)} design(s), {model.nseg} segments, {P} protons, {max_steps} RK4 steps; "
-        f"windings generated on the GPU in {time.perf_counter() - t0:.2f} s")
-
-    s0 = torch.from_numpy(np.ascontiguousarray(state.T)).to(dev)
-    kq_t = torch.from_numpy(kq).to(dev)
-    dt_t = torch.from_numpy(dt).to(dev)
-    ns_t = torch.from_numpy(ns).to(dev)
-    st = torch.empty_like(s0)
-
-    FP32 = 33  # FP32 tracking kernel (K x q - C tile sums)
-    tuned = {}
-    for prec in (FP32, 64):
-        if prec == 64 and (args.no_fp64 or args.roofline_only):
-            continue
-        if args.ppl == 0:
-            tuned[prec] = autotune.tune(model, npt, prec, s0[:3].contiguous())
-        else:
-            ppl = args.ppl if (prec == FP32 or args.ppl > 0) else 2
-            tuned[prec] = {"ppl": ppl, "target_items": args.items_per_cu * engine.cu_count() or None}
-        log(f"[bench] fp{prec} shape: {tuned[prec]}")
+    S = Setup(W.by_name(args.workload))
+    wl, model = S.wl, S.model
+    tuned = {FP32: S.shape(FP32, args.ppl, args.items_per_cu)}
+    log(f"[bench] fp32 shape: {tuned[FP32]}")
 
     def bs_launch_timing(plan, pts, partial, launches: int) -> float:
         """Average duration (ms) of the FP32 Biot-Savart kernel, HIP events on its stream."""
@@ -186,54 +274,35 @@ def main() -> int:
         return e0.elapsed_time(e1) / launches
 
     if args.roofline_only:
-        plan = model.plan(npt, tuned[FP32]["ppl"], tuned[FP32]["target_items"])
+        plan = model.plan(S.npt, tuned[FP32]["ppl"], tuned[FP32]["target_items"])
         _, partial = model.workspace(plan)
-        avg = bs_launch_timing(plan, s0[:3].contiguous(), partial, args.roofline_launches)
+        avg = bs_launch_timing(plan, S.s0[:3].contiguous(), partial, args.roofline_launches)
         print(json.dumps({"workload": wl.name, "kernel": kernel_name(plan.ppl), "work_items": plan.nwork,
                           "launches": args.roofline_launches + 1, "avg_launch_ms": avg}), flush=True)
         return 0
 
-    def run(prec: int, passes: int, warmup: int):
-        plan = model.plan(npt, tuned[prec]["ppl"], tuned[prec]["target_items"])
-        scratch, partial = model.workspace(plan)
-        for _ in range(warmup):
-            st.copy_(s0)
-            model.track_tensors(plan, prec, st, kq_t, dt_t, ns_t, max_steps, scratch, partial)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(passes):
-            st.copy_(s0)
-            model.track_tensors(plan, prec, st, kq_t, dt_t, ns_t, max_steps, scratch, partial)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1)
-        return ms, st.cpu().numpy().T.copy(), plan, partial
-
-    ms32, out32, plan32, partial32 = run(FP32, args.steps, args.warmup)
-    particle_steps = float(ns.sum())
-    seg_of = np.repeat(model.design_nseg, npt)
-    pairs_per_pass = float(np.sum(ns * 4 * seg_of))
-    rate32 = particle_steps * args.steps / (ms32 / 1e3)
-    log(f"[bench] fp32: {ms32 / args.steps:.1f} ms/pass, {rate32:.4g} particle-steps/s")
+    ms32, out32, plan32, partial32 = S.forward(FP32, tuned[FP32], args.steps, args.warmup, args.graph)
+    rate32 = S.particle_steps / (ms32 / 1e3)
+    log(f"[bench] fp32: {ms32:.1f} ms/pass, {rate32:.4g} particle-steps/s")
 
     fp64 = None
     err = None
     if not args.no_fp64:
-        ms64, out64, _, _ = run(64, 1, 1 if args.warmup > 0 else 0)
-        rate64 = particle_steps / (ms64 / 1e3)
+        tuned[64] = S.shape(64, args.ppl, args.items_per_cu)
+        ms64, out64, _, _ = S.forward(64, tuned[64], 1, 1 if args.warmup > 0 else 0, args.graph)
+        rate64 = S.particle_steps / (ms64 / 1e3)
         d = np.linalg.norm(out32[:, :3] - out64[:, :3], axis=1)
         err = {"max": float(d.max() * 1e6), "rms": float(np.sqrt(np.mean(d ** 2)) * 1e6),
                "p99": float(np.percentile(d, 99) * 1e6)}
-        fp64 = {"particle_steps_per_s": rate64, "pair_evals_per_s": pairs_per_pass / (ms64 / 1e3),
+        fp64 = {"particle_steps_per_s": rate64, "pair_evals_per_s": S.pairs_per_pass / (ms64 / 1e3),
                 "ms_per_step": ms64, "ppl": tuned[64]["ppl"]}
         log(f"[bench] fp64: {ms64:.1f} ms/pass, {rate64:.4g} particle-steps/s; fp32 position error "
             f"max {err['max']:.3g} um, rms {err['rms']:.3g} um")
 
     # live timing of the dominant kernel (split-K Biot-Savart, FP32) on the stream it runs on
-    pts = torch.from_numpy(np.ascontiguousarray(out32[:, :3].T)).to(dev)
+    pts = torch.from_numpy(np.ascontiguousarray(out32[:, :3].T)).to(engine.device())
     avg_ms = bs_launch_timing(plan32, pts, partial32, args.roofline_launches)
-    flops = float(np.sum(np.asarray(npt) * model.design_nseg)) * FLOP_PER_PAIR
+    flops = float(np.sum(np.asarray(S.npt) * model.design_nseg)) * FLOP_PER_PAIR
     achieved = flops / (avg_ms / 1e3) / 1e12
     kname = kernel_name(plan32.ppl)
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[32], "unit": "TFLOP/s",
@@ -243,6 +312,33 @@ def main() -> int:
                                                       + plan32.nwork * 3 * engine.points_per_block(plan32.ppl) * 8)}
     log(f"[bench] bs kernel: {avg_ms * 1e3:.1f} us/launch, {achieved:.2f} TFLOP/s "
         f"({100 * roofline['frac']:.1f}% of FP32 vector peak)")
+
+    extras = None
+    if not args.no_extras:
+        extras = {}
+        P_ = Setup(W.by_name("population"), align_cct=True)
+        sh = P_.shape(FP32, 0, 0)
+        pm32, pout32, _, _ = P_.forward(FP32, sh, 2, 1, True)
+        sh64 = P_.shape(64, 0, 0)
+        pm64, pout64, _, _ = P_.forward(64, sh64, 1, 1, True)
+        d = np.linalg.norm(pout32[:, :3] - pout64[:, :3], axis=1)
+        fa = P_.forward_adjoint(sh, 1, 1)
+        extras["population"] = {
+            "designs": len(P_.bls), "segments": int(P_.model.nseg), "protons": P_.P, "rk4_steps_max": P_.max_steps,
+            "fp32_particle_steps_per_s": P_.particle_steps / (pm32 / 1e3), "fp32_ms_per_pass": pm32,
+            "fp32_shape": sh, "fp64_particle_steps_per_s": P_.particle_steps / (pm64 / 1e3), "fp64_ms_per_pass": pm64,
+            "fp32_vs_fp64": pm64 / pm32, "position_error_um_vs_fp64": {"max": float(d.max() * 1e6),
+                                                                       "rms": float(np.sqrt(np.mean(d ** 2)) * 1e6)},
+            "hip_graph": True, "forward_adjoint": fa,
+            "forward_adjoint_vs_fp64_forward": fa["particle_steps_per_s"] / (P_.particle_steps / (pm64 / 1e3))}
+        log(f"[bench] population: fp32 {pm32:.1f} ms, fp64 {pm64:.1f} ms, fwd+adj {fa['ms_per_pass']:.1f} ms, "
+            f"error max {d.max() * 1e6:.3g} um")
+        del P_
+        A_ = Setup(W.by_name("adjoint_section"), align_cct=True)
+        fa = A_.forward_adjoint(A_.shape(FP32, 0, 0), 1, 1)
+        extras["adjoint_section"] = dict(protons=A_.P, segments=int(A_.model.nseg), rk4_steps=A_.max_steps, **fa)
+        log(f"[bench] adjoint_section: fwd+adj {fa['ms_per_pass']:.1f} ms ({fa['forward_ms']:.1f} forward)")
+        del A_
 
     cpu = None
     if not args.no_cpu:
@@ -257,22 +353,23 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms32 / args.steps,
+        "ms_per_step": ms32,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 Biot-Savart terms; f64 tile sums and RK4 state",
         "data": "synthetic",
-        "config": {"workload": wl.name, "designs": len(bls), "segments": int(model.nseg), "protons": P,
-                   "rk4_steps": max_steps, "footstep_mm": wl.footstep * 1e3, "length_m": wl.length,
-                   "parallelism": "single GPU", "ppl": plan32.ppl, "work_items": plan32.nwork,
+        "config": {"workload": wl.name, "designs": len(S.bls), "segments": int(model.nseg), "protons": S.P,
+                   "rk4_steps": S.max_steps, "footstep_mm": wl.footstep * 1e3, "length_m": wl.length,
+                   "parallelism": "single GPU", "ppl": plan32.ppl, "work_items": plan32.nwork, "hip_graph": args.graph,
                    "target_items": tuned[FP32]["target_items"]},
-        "pair_evals_per_s": pairs_per_pass * args.steps / (ms32 / 1e3),
+        "pair_evals_per_s": S.pairs_per_pass / (ms32 / 1e3),
         "fp64_plain": fp64,
         "speedup_vs_fp64": (rate32 / fp64["particle_steps_per_s"]) if fp64 else None,
         "position_error_um_vs_fp64": err,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "extras": extras,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -40,24 +40,76 @@ class AdjointTracker:
         self.ckpt_every = int(ckpt_every)
         self.ppl = ppl
 
-    def forward(self, state: numpy.ndarray, kq: numpy.ndarray, dt: numpy.ndarray, nsteps: numpy.ndarray,
-                design_npt: Sequence[int]) -> numpy.ndarray:
-        """Track (P, 6) states; keeps the checkpoints for ``backward``."""
+    def prepare(self, design_npt: Sequence[int], kq: torch.Tensor, dt: torch.Tensor, nsteps: torch.Tensor,
+                max_steps: int) -> None:
+        """Plan and device workspaces for particles grouped by design; kq, dt, nsteps are
+        device tensors (P,) that stay bound to this tracker."""
         dev = engine.device()
         m = self.model
         self.plan = plan = m.plan(design_npt, self.ppl)
         self.P = P = plan.npt
-        self.nsteps = numpy.asarray(nsteps, numpy.int64)
-        self.max_steps = int(self.nsteps.max())
-        self.kq = torch.from_numpy(numpy.ascontiguousarray(kq, numpy.float64)).to(dev)
-        self.dt = torch.from_numpy(numpy.ascontiguousarray(dt, numpy.float64)).to(dev)
-        self.ns = torch.from_numpy(numpy.ascontiguousarray(self.nsteps)).to(dev)
-        st = torch.from_numpy(numpy.ascontiguousarray(numpy.asarray(state, numpy.float64).reshape(-1, 6).T)).to(dev)
+        self.kq, self.dt, self.ns = kq, dt, nsteps
+        self.max_steps = int(max_steps)
         nck = -(-self.max_steps // self.ckpt_every)
         self.ckpt = torch.zeros((nck + 1, 6, P), dtype=torch.float64, device=dev)
-        scratch, partial = m.workspace(plan)
-        m.track_tensors(plan, self.precision, st, self.kq, self.dt, self.ns, self.max_steps, scratch, partial,
-                        record=self.ckpt, record_every=self.ckpt_every)
+        self.scratch, self.partial = m.workspace(plan)
+        self.ntile = max(m.nseg_padded // N.TILE, 1)
+        pb = engine.points_per_block(plan.ppl)
+        nblock = max(-(-int(n) // pb) for n in numpy.diff(plan.design_pt_off.cpu().numpy()))
+        self.tgrad = torch.zeros((max(nblock, 1), self.ntile, 5), dtype=torch.float64, device=dev)
+        self.nq = nq = int(m.design_qs_off_host[-1])
+        self.qgrad = torch.zeros((max(2 * nq, 1), P), dtype=torch.float64, device=dev)
+        self.state_ws = torch.empty((6, P), dtype=torch.float64, device=dev)
+        self.tape = torch.zeros((self.ckpt_every, 36, P), dtype=torch.float64, device=dev)
+        self.adj_ws = torch.zeros((15, P), dtype=torch.float64, device=dev)
+        self.bs_pts = torch.zeros((3, P), dtype=torch.float64, device=dev)
+        self.bs_wts = torch.zeros((3, P), dtype=torch.float64, device=dev)
+
+    def forward_tensors(self, state: torch.Tensor) -> None:
+        """In-place tracking of device state (6, P), keeping checkpoints every K steps."""
+        self.model.track_tensors(self.plan, self.precision, state, self.kq, self.dt, self.ns, self.max_steps,
+                                 self.scratch, self.partial, record=self.ckpt, record_every=self.ckpt_every)
+
+    def backward_tensors(self, adj: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+        """In-place reverse pass: adj (6, P) dJ/d(final state) -> dJ/d(initial state).
+        Returns the device accumulators (tgrad (nblock, ntile, 5), qgrad (2Q, P))."""
+        m, plan = self.model, self.plan
+        seg, _ = m._seg(self.precision)
+        org = m.tile_org  # FP32 tile origins: the backward kernel needs them at every precision
+        tan, tile_cct, coef = m.adjoint_data()
+        self.tgrad.zero_()
+        self.qgrad.zero_()
+        N.check(N.lib.cctamd_track_rk4_adjoint(
+            self.precision, plan.ppl, _p(seg), _p(org), _p(m.seg32), _p(tan), _p(tile_cct), _p(coef), self.ntile,
+            _p(plan.work), plan.nwork, _p(plan.wbase), _p(plan.nchunk), _p(plan.design_pt_off), m.ndesign,
+            _p(m.qs), _p(m.design_qs_off), _p(m.bconst), _p(self.ckpt), self.ckpt_every, self.P, _p(self.kq),
+            _p(self.dt), _p(self.ns), self.max_steps, _p(adj), _p(self.tgrad), _p(self.qgrad), _p(self.state_ws),
+            _p(self.scratch), _p(self.partial), _p(self.tape), _p(self.adj_ws), _p(self.bs_pts), _p(self.bs_wts),
+            _stream()), "cctamd_track_rk4_adjoint")
+        return self.tgrad, self.qgrad
+
+    def reduce_gradients(self) -> Tuple[numpy.ndarray, numpy.ndarray]:
+        """(cct_grad (ncct, 5) per global CCT: [sum B-bar.dB, dJ/dc_0..3], qs_grad (Q, 2))."""
+        m = self.model
+        _, tile_cct, _ = m.adjoint_data()
+        tg = self.tgrad.sum(0).cpu().numpy()
+        tc = tile_cct.cpu().numpy()
+        cct_grad = numpy.zeros((m.ncct, 5))
+        for c in range(m.ncct):
+            cct_grad[c] = tg[tc == c].sum(0)
+        q = self.qgrad.cpu().numpy()[: 2 * self.nq].reshape(self.nq, 2, self.P).sum(2)
+        return cct_grad, q
+
+    def forward(self, state: numpy.ndarray, kq: numpy.ndarray, dt: numpy.ndarray, nsteps: numpy.ndarray,
+                design_npt: Sequence[int]) -> numpy.ndarray:
+        """Track (P, 6) states; keeps the checkpoints for ``backward``."""
+        dev = engine.device()
+        ns = numpy.asarray(nsteps, numpy.int64)
+        self.prepare(design_npt, torch.from_numpy(numpy.ascontiguousarray(kq, numpy.float64)).to(dev),
+                     torch.from_numpy(numpy.ascontiguousarray(dt, numpy.float64)).to(dev),
+                     torch.from_numpy(numpy.ascontiguousarray(ns)).to(dev), int(ns.max()))
+        st = torch.from_numpy(numpy.ascontiguousarray(numpy.asarray(state, numpy.float64).reshape(-1, 6).T)).to(dev)
+        self.forward_tensors(st)
         return st.cpu().numpy().T.copy()
 
     def backward(self, adj_final: numpy.ndarray):
@@ -67,34 +119,9 @@ class AdjointTracker:
         [sum B-bar.dB, dJ/dc_0..3], qs_grad (Q, 2)).
         """
         dev = engine.device()
-        m, plan, P = self.model, self.plan, self.P
-        seg, _ = m._seg(self.precision)
-        org = m.tile_org  # FP32 tile origins: the backward kernel needs them at every precision
-        tan, tile_cct, coef = m.adjoint_data()
-        ntile = max(m.nseg_padded // N.TILE, 1)
-        nblock = max(-(-int(n) // engine.points_per_block(plan.ppl)) for n in numpy.diff(plan.design_pt_off.cpu().numpy()))
         adj = torch.from_numpy(numpy.ascontiguousarray(numpy.asarray(adj_final, numpy.float64).reshape(-1, 6).T)).to(dev)
-        tgrad = torch.zeros((max(nblock, 1), ntile, 5), dtype=torch.float64, device=dev)
-        nq = int(m.design_qs_off_host[-1])
-        qgrad = torch.zeros((max(2 * nq, 1), P), dtype=torch.float64, device=dev)
-        state_ws = torch.empty((6, P), dtype=torch.float64, device=dev)
-        scratch, partial = m.workspace(plan)
-        tape = torch.zeros((self.ckpt_every, 36, P), dtype=torch.float64, device=dev)
-        adj_ws = torch.zeros((15, P), dtype=torch.float64, device=dev)
-        bs_pts = torch.zeros((3, P), dtype=torch.float64, device=dev)
-        bs_wts = torch.zeros((3, P), dtype=torch.float64, device=dev)
-        N.check(N.lib.cctamd_track_rk4_adjoint(
-            self.precision, plan.ppl, _p(seg), _p(org), _p(m.seg32), _p(tan), _p(tile_cct), _p(coef), ntile,
-            _p(plan.work), plan.nwork, _p(plan.wbase), _p(plan.nchunk), _p(plan.design_pt_off), m.ndesign,
-            _p(m.qs), _p(m.design_qs_off), _p(m.bconst), _p(self.ckpt), self.ckpt_every, P, _p(self.kq), _p(self.dt),
-            _p(self.ns), self.max_steps, _p(adj), _p(tgrad), _p(qgrad), _p(state_ws), _p(scratch), _p(partial),
-            _p(tape), _p(adj_ws), _p(bs_pts), _p(bs_wts), _stream()), "cctamd_track_rk4_adjoint")
-        tg = tgrad.sum(0).cpu().numpy()
-        tc = tile_cct.cpu().numpy()
-        cct_grad = numpy.zeros((m.ncct, 5))
-        for c in range(m.ncct):
-            cct_grad[c] = tg[tc == c].sum(0)
-        q = qgrad.cpu().numpy()[: 2 * nq].reshape(nq, 2, P).sum(2)
+        self.backward_tensors(adj)
+        cct_grad, q = self.reduce_gradients()
         return adj.cpu().numpy().T.copy(), cct_grad, q
 
 

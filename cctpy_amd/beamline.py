@@ -37,6 +37,18 @@ class Beamline(Line2, Magnet, ApertureObject):
                                 step: float = 1 * MM) -> List[P2]:
         return Magnet.magnetic_field_bz_along(self, self.trajectory if line2 is None else line2, p2_t0_p3, step)
 
+    def graident_field_along(self, line2: Optional[Line2] = None, good_field_area_width: float = 10 * MM,
+                             step: float = 1 * MM, point_number: int = 4) -> List[P2]:
+        """Magnet.graident_field_along on the beamline's own orbit by default (beamline.py:96-115)."""
+        return Magnet.graident_field_along(self, self.trajectory if line2 is None else line2, good_field_area_width,
+                                           step, point_number)
+
+    def second_graident_field_along(self, line2: Optional[Line2] = None, good_field_area_width: float = 10 * MM,
+                                    step: float = 1 * MM, point_number: int = 4) -> List[P2]:
+        """Magnet.second_graident_field_along on the orbit by default (beamline.py:117-136)."""
+        return Magnet.second_graident_field_along(self, self.trajectory if line2 is None else line2,
+                                                  good_field_area_width, step, point_number)
+
     # --------------------------------------------------------- tracking
     def track_ideal_particle(self, kinetic_MeV: float, s: float = 0.0, length: Optional[float] = None,
                              footstep: float = 5 * MM) -> List[P3]:

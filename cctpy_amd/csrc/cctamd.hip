@@ -84,7 +84,7 @@ __global__ void k_pack_fp32(const double* __restrict__ seg64, int64_t nseg, floa
     d[6] = (float)(s[6] - o[2]);
     d[7] = 0.0f;
     if (seg32k) {
-        // {kl, kl x (m - c), m - c} rounded once from FP64
+        // {kl, kl x (m - c), -2 (m - c), |m - c|^2}, each rounded once from FP64
         const double mx = s[4] - o[0], my = s[5] - o[1], mz = s[6] - o[2];
         float* e = seg32k + i * 12;
         e[0] = (float)s[0];
@@ -93,16 +93,26 @@ __global__ void k_pack_fp32(const double* __restrict__ seg64, int64_t nseg, floa
         e[3] = (float)(s[1] * mz - s[2] * my);
         e[4] = (float)(s[2] * mx - s[0] * mz);
         e[5] = (float)(s[0] * my - s[1] * mx);
-        e[6] = (float)mx;
-        e[7] = (float)my;
-        e[8] = (float)mz;
-        e[9] = e[10] = e[11] = 0.0f;
+        e[6] = (float)(-2.0 * mx);
+        e[7] = (float)(-2.0 * my);
+        e[8] = (float)(-2.0 * mz);
+        e[9] = (float)(mx * mx + my * my + mz * mz);
+        e[10] = e[11] = 0.0f;
     }
     if (i % kTile == 0) {
+        // extent of the tile: max |m - c|^2 over its current-carrying rows (pads excluded)
+        double ext = 0.0;
+        for (int k = 0; k < kTile && i + k < nseg; ++k) {
+            const double* r = seg64 + (i + k) * 8;
+            if (r[0] == 0.0 && r[1] == 0.0 && r[2] == 0.0) continue;
+            const double dx = r[4] - o[0], dy = r[5] - o[1], dz = r[6] - o[2];
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            ext = d2 > ext ? d2 : ext;
+        }
         tile_org[t * 4 + 0] = o[0];
         tile_org[t * 4 + 1] = o[1];
         tile_org[t * 4 + 2] = o[2];
-        tile_org[t * 4 + 3] = 0.0;
+        tile_org[t * 4 + 3] = ext;
     }
 }
 
@@ -166,8 +176,26 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32(const float* __restrict_
 
 // FP32 tracking variant: kl x (q - m) = kl x q - kl x m, so within a tile the
 // sums K = sum w kl and C = sum w (kl x m) (w = |q - m|^-3) give the tile field
-// K x q - C with 14 VALU operations per pair instead of 17.  Points must stay
-// well away from the conductors (a tile spans a few cm): true inside a bore.
+// K x q - C; and |q - m|^2 = (|q|^2 + |m|^2) - 2 q.m needs no difference
+// vector.  13 VALU slots per pair (add, 3 fma, rsq, 2 mul, 6 fma) instead of 17.
+// |q|^2 is formed in FP64 per tile.  Both rewrites cancel when the point is
+// close to the tile (|q| of the order of the tile extent e): for |q| >= 4 e the
+// cancellation factors stay below 2, so a wave that has any lane with
+// |q|^2 < 16 e^2 (a point near or inside the windings) sums that tile in the
+// direct form (kl x r) |r|^-3 instead.
+constexpr double kNearTile = 16.0;
+
+__device__ inline void tile_direct_term(const float4& a, const float4& b, const float4& c, float qx, float qy,
+                                        float qz, float& bx, float& by, float& bz) {
+    // m = -0.5 (-2m) is exact in FP32
+    const float rx = __builtin_fmaf(0.5f, b.z, qx), ry = __builtin_fmaf(0.5f, b.w, qy), rz = __builtin_fmaf(0.5f, c.x, qz);
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float w3 = inv * inv * inv;
+    bx += (a.y * rz - a.z * ry) * w3;
+    by += (a.z * rx - a.x * rz) * w3;
+    bz += (a.x * ry - a.y * rx) * w3;
+}
 template <int PPL>
 __global__ __launch_bounds__(kWG) void k_bs_partial_f32k(const float* __restrict__ seg32k,
                                                           const double* __restrict__ tile_org,
@@ -187,24 +215,44 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k(const float* __restrict
     const int tend = w.tile_start + w.tile_count;
     for (int t = w.tile_start + wave; t < tend; t += kWavesPerWG) {
         const double ox = tile_org[4 * t], oy = tile_org[4 * t + 1], oz = tile_org[4 * t + 2];
-        float qx[PPL], qy[PPL], qz[PPL], kx[PPL], ky[PPL], kz[PPL], cx[PPL], cy[PPL], cz[PPL];
+        const double near2 = kNearTile * tile_org[4 * t + 3];
+        float qx[PPL], qy[PPL], qz[PPL], qq[PPL], kx[PPL], ky[PPL], kz[PPL], cx[PPL], cy[PPL], cz[PPL];
+        bool near = false;
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
-            qx[j] = (float)(px[j] - ox);
-            qy[j] = (float)(py[j] - oy);
-            qz[j] = (float)(pz[j] - oz);
+            const double dx = px[j] - ox, dy = py[j] - oy, dz = pz[j] - oz;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            near = near || d2 < near2;
+            qx[j] = (float)dx;
+            qy[j] = (float)dy;
+            qz[j] = (float)dz;
+            qq[j] = (float)d2;
             kx[j] = ky[j] = kz[j] = cx[j] = cy[j] = cz[j] = 0.0f;
         }
         const float4* s = reinterpret_cast<const float4*>(seg32k) + (int64_t)t * kTile * 3;
+        if (__any(near)) {  // wave-uniform: direct form for the whole wave on this tile
+#pragma unroll 1
+            for (int k = 0; k < kTile; ++k) {
+                const float4 a = s[3 * k], b = s[3 * k + 1], c = s[3 * k + 2];
+#pragma unroll
+                for (int j = 0; j < PPL; ++j) tile_direct_term(a, b, c, qx[j], qy[j], qz[j], kx[j], ky[j], kz[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                ax[j] += (double)kx[j];
+                ay[j] += (double)ky[j];
+                az[j] += (double)kz[j];
+            }
+            continue;
+        }
 #pragma unroll 8
         for (int k = 0; k < kTile; ++k) {
             const float4 a = s[3 * k];      // kl.xyz, (kl x m).x
-            const float4 b = s[3 * k + 1];  // (kl x m).yz, m.xy
-            const float4 c = s[3 * k + 2];  // m.z
+            const float4 b = s[3 * k + 1];  // (kl x m).yz, -2m.xy
+            const float4 c = s[3 * k + 2];  // -2m.z, |m|^2
 #pragma unroll
             for (int j = 0; j < PPL; ++j) {
-                const float rx = qx[j] - b.z, ry = qy[j] - b.w, rz = qz[j] - c.x;
-                const float r2 = rx * rx + ry * ry + rz * rz;
+                const float r2 = __builtin_fmaf(qz[j], c.x, __builtin_fmaf(qy[j], b.w, __builtin_fmaf(qx[j], b.z, qq[j] + c.y)));
                 const float inv = __builtin_amdgcn_rsqf(r2);
                 const float w3 = inv * inv * inv;
                 kx[j] += a.x * w3;
@@ -254,29 +302,40 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __rest
     double ax = 0.0, ay = 0.0, az = 0.0;
     const int tend = w.tile_start + w.tile_count;
     const float4* src = reinterpret_cast<const float4*>(seg32k);
+    // missing tiles: zero current, |m|^2 = 1e6 keeps |q - m|^2 > 0
     const float4 pad0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float4 pad1 = make_float4(0.0f, 0.0f, 1.0e3f, 1.0e3f);
-    const float4 pad2 = make_float4(1.0e3f, 0.0f, 0.0f, 0.0f);
+    const float4 pad2 = make_float4(0.0f, 1.0e6f, 0.0f, 0.0f);
     for (int t0 = w.tile_start; t0 < tend; t0 += NS) {
         for (int i = threadIdx.x; i < NS * ROW; i += kWG) {
             const int s = i / ROW, e = i % ROW;
             const int t = t0 + s;
-            lds[s][e] = t < tend ? src[(int64_t)t * ROW + e] : (e % 3 == 0 ? pad0 : (e % 3 == 1 ? pad1 : pad2));
+            lds[s][e] = t < tend ? src[(int64_t)t * ROW + e] : (e % 3 == 2 ? pad2 : pad0);
         }
         __syncthreads();
-        const int t = t0 + stream < tend ? t0 + stream : w.tile_start;
-        const float qx = (float)(px - tile_org[4 * t]);
-        const float qy = (float)(py - tile_org[4 * t + 1]);
-        const float qz = (float)(pz - tile_org[4 * t + 2]);
+        const bool real = t0 + stream < tend;
+        const int t = real ? t0 + stream : w.tile_start;
+        const double dx = px - tile_org[4 * t], dy = py - tile_org[4 * t + 1], dz = pz - tile_org[4 * t + 2];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        const float qx = (float)dx, qy = (float)dy, qz = (float)dz;
+        const float qq = (float)d2;
         float kx = 0.0f, ky = 0.0f, kz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
         const float4* r4 = lds[stream];
+        if (__any(real && d2 < kNearTile * tile_org[4 * t + 3])) {  // direct form for this round
+#pragma unroll 1
+            for (int k = 0; k < kTile; ++k)
+                tile_direct_term(r4[3 * k], r4[3 * k + 1], r4[3 * k + 2], qx, qy, qz, kx, ky, kz);
+            ax += (double)kx;
+            ay += (double)ky;
+            az += (double)kz;
+            __syncthreads();
+            continue;
+        }
 #pragma unroll 8
         for (int k = 0; k < kTile; ++k) {
             const float4 a = r4[3 * k];
             const float4 b = r4[3 * k + 1];
             const float4 c = r4[3 * k + 2];
-            const float rx = qx - b.z, ry = qy - b.w, rz = qz - c.x;
-            const float r2 = rx * rx + ry * ry + rz * rz;
+            const float r2 = __builtin_fmaf(qz, c.x, __builtin_fmaf(qy, b.w, __builtin_fmaf(qx, b.z, qq + c.y)));
             const float inv = __builtin_amdgcn_rsqf(r2);
             const float w3 = inv * inv * inv;
             kx += a.x * w3;

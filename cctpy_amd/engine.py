@@ -331,6 +331,30 @@ class DeviceModel:
         return self.seg64.cpu().numpy()[: self.nseg_padded]
 
 
+class TrackGraph:
+    """One tracking pass (every RK4 step of a plan: 8 launches per step) captured
+    once as a HIP graph and replayed.
+
+    The graph binds the device buffers given here: refresh ``state`` in place
+    (``state.copy_(initial)``) before each replay.  The captured launches hold
+    no memset nodes and no host synchronisation.
+    """
+
+    def __init__(self, model: DeviceModel, plan: Plan, precision: int, state: torch.Tensor, kq: torch.Tensor,
+                 dt: torch.Tensor, nsteps: torch.Tensor, max_steps: int, scratch: torch.Tensor,
+                 partial: torch.Tensor):
+        self.buffers = (state, kq, dt, nsteps, scratch, partial)  # kept alive with the graph
+        self.graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()  # default priority
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.graph(self.graph, stream=side):
+            model.track_tensors(plan, precision, state, kq, dt, nsteps, max_steps, scratch, partial)
+        torch.cuda.current_stream().wait_stream(side)
+
+    def replay(self) -> None:
+        self.graph.replay()
+
+
 _CACHE: "collections.OrderedDict[bytes, DeviceModel]" = collections.OrderedDict()
 _CACHE_SIZE = 8
 

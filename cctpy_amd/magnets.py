@@ -52,6 +52,86 @@ class Magnet:
                                 step: float = 1 * MM) -> List[P2]:
         return [P2(v.distance, v.value.z) for v in self.magnetic_field_along(line2, p2_t0_p3=p2_t0_p3, step=step)]
 
+    # -- field analysis along a line (magnets.py:112-390) --------------------
+    def _transverse_bz(self, side_points: Callable[[float], Tuple[P3, P3]], length: float,
+                       good_field_area_width: float, step: float, point_number: int):
+        """Bz on point_number points of the transverse segment (right -> left) at every
+        s = linspace(0, length, int(length/step)+1): one GPU launch for all points."""
+        xs = BaseUtils.linspace(-good_field_area_width / 2, good_field_area_width / 2, point_number)
+        distances = BaseUtils.linspace(0, length, int(length / step) + 1)
+        pts = []
+        for s in distances:
+            right, left = side_points(s)
+            pts += BaseUtils.linspace(right, left, point_number)
+        bz = numpy.array([b.z for b in self.magnetic_field_at_points(pts)]).reshape(len(distances), point_number)
+        return xs, distances, bz
+
+    def _line2_sides(self, line2: Line2, good_field_area_width: float):
+        d = good_field_area_width / 2
+        return lambda s: (line2.right_hand_side_point(s=s, d=d).to_p3(), line2.left_hand_side_point(s=s, d=d).to_p3())
+
+    @staticmethod
+    def _fit(xs, bz: numpy.ndarray, order: int) -> numpy.ndarray:
+        """numpy.polyfit of every row (BaseUtils.polynomial_fitting), constant term first."""
+        if bz.shape[0] == 0:
+            return numpy.zeros((0, order + 1))
+        return numpy.polyfit(numpy.asarray(xs), bz.T, order)[::-1].T
+
+    def graident_field_along(self, line2: Line2, good_field_area_width: float = 10 * MM, step: float = 1 * MM,
+                             point_number: int = 4) -> List[P2]:
+        """Quadrupole gradient along line2 from a linear fit of Bz across it (magnets.py:112-164)."""
+        xs, ds, bz = self._transverse_bz(self._line2_sides(line2, good_field_area_width), line2.get_length(),
+                                         good_field_area_width, step, point_number)
+        c = self._fit(xs, bz, 1)
+        return [P2(s, float(c[i, 1])) for i, s in enumerate(ds)]
+
+    def second_graident_field_along(self, line2: Line2, good_field_area_width: float = 10 * MM,
+                                    step: float = 1 * MM, point_number: int = 4) -> List[P2]:
+        """Sextupole gradient: 2 x the quadratic coefficient of the fit (magnets.py:166-219)."""
+        xs, ds, bz = self._transverse_bz(self._line2_sides(line2, good_field_area_width), line2.get_length(),
+                                         good_field_area_width, step, point_number)
+        c = self._fit(xs, bz, 2)
+        return [P2(s, float(c[i, 2]) * 2.0) for i, s in enumerate(ds)]
+
+    @staticmethod
+    def _multipoles(c: numpy.ndarray, ds) -> List[ValueWithDistance[List[float]]]:
+        out = []
+        for i, s in enumerate(ds):
+            g = c[i].tolist()
+            for k in range(2, len(g)):
+                g[k] = g[k] * math.factorial(k)
+            out.append(ValueWithDistance(value=g, distance=s))
+        return out
+
+    def multipole_field_along(self, line2: Line2, order: int, good_field_area_width: float = 10 * MM,
+                              step: float = 1 * MM, point_number: int = 10) -> List[ValueWithDistance[List[float]]]:
+        """[dipole, quadrupole, sextupole, ...] up to ``order`` along line2; the i-th fit
+        coefficient times i! (magnets.py:221-287)."""
+        xs, ds, bz = self._transverse_bz(self._line2_sides(line2, good_field_area_width), line2.get_length(),
+                                         good_field_area_width, step, point_number)
+        return self._multipoles(self._fit(xs, bz, order), ds)
+
+    def multipole_field_along_line3(self, line3, order: int, good_field_area_width: float = 10 * MM,
+                                    step: float = 1 * MM, point_number: int = 10,
+                                    plane_direct: P3 = None) -> List[ValueWithDistance[List[float]]]:
+        """As ``multipole_field_along`` on a 3D curve, transverse in the plane normal to
+        ``plane_direct`` (default global z) (magnets.py:289-350)."""
+        pd = P3.z_direct() if plane_direct is None else plane_direct
+        d = good_field_area_width / 2
+        sides = lambda s: (line3.right_hand_side_point(s=s, d=d, plane_direct=pd),  # noqa: E731
+                           line3.left_hand_side_point(s=s, d=d, plane_direct=pd))
+        xs, ds, bz = self._transverse_bz(sides, line3.get_length(), good_field_area_width, step, point_number)
+        return self._multipoles(self._fit(xs, bz, order), ds)
+
+    def integration_field(self, line2: Line2, p2_t0_p3: Callable[[P2], P3] = lambda p2: P3(p2.x, p2.y, 0.0),
+                          step: float = 1 * MM) -> float:
+        """Trapezoid integral of Bz along line2 with the sampling step (magnets.py:352-374)."""
+        fields = self.magnetic_field_bz_along(line2, p2_t0_p3, step)
+        ret = 0.0
+        for i in range(len(fields) - 1):
+            ret += (fields[i].y + fields[i + 1].y) / 2 * step
+        return ret
+
     @staticmethod
     def no_magnet() -> "Magnet":
         return UniformMagnet(P3.zeros())

@@ -41,7 +41,8 @@ from typing import Dict, List, Sequence, Tuple
 import numpy
 
 MM = 0.001
-SEEDS = {"single_cct": 20261201, "agcct_section": 20261202, "population": 20261203, "hust_total": 20261205}
+SEEDS = {"single_cct": 20261201, "agcct_section": 20261202, "population": 20261203, "adjoint_section": 20261204,
+         "hust_total": 20261205}
 
 Op = Tuple[str, Dict]
 
@@ -164,9 +165,18 @@ def hust_total(n_particles: int = 4096, seed: int = SEEDS["hust_total"], length:
                     footstep)
 
 
+def adjoint_section(n_particles: int = 256, seed: int = SEEDS["adjoint_section"]) -> Workload:
+    """The AG-CCT section at its design vector plus a 1.1 m drift to the objective plane
+    (rms x + rms y + |dispersion| there, differentiated w.r.t. the 10-parameter vector)."""
+    rng = numpy.random.default_rng(seed)
+    length = agcct_section(n_particles=1).length + 1.1
+    return Workload("adjoint_section", [section_ops(section_theta0())],
+                    [Beam(gaussian_beam(rng, n_particles, 0.08), 0.0, 250.0)], length, 1 * MM)
+
+
 def by_name(name: str, **kw) -> Workload:
     return {"single_cct": single_cct, "agcct_section": agcct_section, "population": population,
-            "hust_total": hust_total}[name](**kw)
+            "adjoint_section": adjoint_section, "hust_total": hust_total}[name](**kw)
 
 
 # ------------------------------------------------------------- builders

@@ -70,8 +70,8 @@ int cctamd_winding_points(const double *params, int64_t npoints, double *out, vo
 
 /*
  * FP32 tile packing of a FP64 segment set (nseg a multiple of CCTAMD_TILE).
- * seg32k (nullable) receives the 12-float rows {kl, kl x (m-c), m-c, 0, 0, 0} of
- * precision 33 (see cctamd_bs_field).
+ * seg32k (nullable) receives the 12-float rows {kl, kl x (m-c), -2(m-c), |m-c|^2, 0, 0}
+ * of precision 33 (see cctamd_bs_field).
  */
 int cctamd_pack_fp32(const double *seg64, int64_t nseg, float *seg32, double *tile_org, float *seg32k,
                      void *stream);
@@ -100,8 +100,9 @@ int64_t cctamd_plan(int64_t ndesign, const int64_t *design_ntile, const int64_t 
  * beamline.py:46-53).
  *   precision: 64 (seg64, FP64 sum), 32 (seg32 + tile_org: FP32 terms of the
  *              re-centred tile, FP64 accumulation across tiles) or 33 (seg32k +
- *              tile_org: the same with per-tile sums K = sum w kl, C = sum w kl x m and
- *              B_tile = K x q - C; for points away from the windings, e.g. in a bore).
+ *              tile_org: the same with per-tile sums K = sum w kl, C = sum w kl x m,
+ *              B_tile = K x q - C and |q-m|^2 = (|q|^2 + |m|^2) - 2 q.m; for points away
+ *              from the windings, e.g. in a bore).
  *   partial: device workspace of nwork*3*PB doubles (PB = points per block); design d's
  *            partials start at design_wbase[d]*3*PB, laid out [block][point][component][chunk].
  */
@@ -181,6 +182,38 @@ int cctamd_track_rk4_adjoint(int precision, int ppl, const void *seg, const doub
                              double *adj, double *tgrad, double *qgrad, double *state_ws, double *scratch,
                              double *partial, double *tape, double *adj_ws, double *bs_pts, double *bs_wts,
                              void *stream);
+
+/*
+ * Phase-space projection of tracked particles and beam statistics per group.
+ * Replaces PhaseSpaceParticle.create_from_running_particle(s)
+ * (final_code/packages/particles.py:1077-1130) applied to every design of a
+ * population, and the beam-size statistics of the design loop
+ * (cctpy3/run.py:72-103: (max - min) / 2 of x and y per group of particles).
+ *   ideal: double[ndesign][16] = {p.xyz, v.xyz, P (speed * relativistic mass),
+ *          XI.xyz, YI.xyz, ZI.xyz} of the ideal particle and its natural frame;
+ *   state: double[6][P] tracked states; speed[P], momentum[P]: each particle's
+ *          scalar speed and speed * relativistic mass (RunningParticle fields);
+ *   group_off[G+1] (particle ranges, clamped to [0, P]) and group_design[G]
+ *          (index into ideal, clamped to [0, ndesign)), device arrays;
+ *   pps: double[6][P] out, rows x, xp, y, yp, z, delta;
+ *   stats: double[G][12] out = {n, min x, max x, min y, max y, mean x, mean y,
+ *          mean delta, rms x, rms y, sum (delta - mean)^2, sum (x - mean x)(delta - mean delta)}.
+ * Reductions run in a fixed order (deterministic).
+ */
+int cctamd_phase_space(const double *ideal, int64_t ndesign, const double *state, const double *speed,
+                       const double *momentum, int64_t nparticles, const int64_t *group_off,
+                       const int32_t *group_design, int64_t ngroups, double *pps, double *stats,
+                       void *stream);
+
+/*
+ * Adjoint seed of the beam-spot objective J_g = w0 rms x + w1 rms y + w2 |D| of
+ * each group (D = dispersion, stats[11] / stats[10]; no counterpart in the
+ * reference, which has no gradients): writes dJ/d(final x, v) into adj[6][P] for
+ * the particles of every group (the velocity rows are zero).  weights: host double[3].
+ */
+int cctamd_beam_objective_seed(const double *ideal, int64_t ndesign, const double *pps, const double *stats,
+                               int64_t nparticles, const int64_t *group_off, const int32_t *group_design,
+                               int64_t ngroups, const double *weights, double *adj, void *stream);
 
 #ifdef __cplusplus
 }

@@ -110,6 +110,31 @@ def test_field_fp32_kc_in_bore_accuracy():
         assert err.max() < 1e-6, (prec, err.max())
 
 
+@pytest.mark.parametrize("ppl", [2, -2])
+def test_field_fp32_kc_next_to_windings(ppl):
+    """Precision 33 at points 2-6 mm from the conductors (a beam that leaves the bore): the
+    tiles near a point are summed in the direct form, so the error stays at the FP32 level."""
+    from cctpy_amd import workloads as W
+    from oracle import workload_builder as WB
+    wl = W.agcct_section(n_particles=8, disperse_number_per_winding=180)
+    bl = W.build_beamline(wl.designs[0])
+    ref = WB.build_beamline(wl.designs[0])
+    ccts = [m for m in ref.magnets if isinstance(m, O.CCTMagnet)]
+    rng = np.random.default_rng(9)
+    pts = []
+    for c in ccts:
+        _, mids = c.global_segments()
+        for i in rng.integers(0, len(mids), 12):
+            d = rng.normal(size=3)
+            pts.append(mids[i] + d / np.linalg.norm(d) * rng.uniform(2e-3, 6e-3))
+    pts = np.array(pts)
+    model = engine.model_for([device_parts(bl).spec()])
+    want = C.field(O.beamline_arrays(ref.magnets), pts)
+    b = model.field(pts, [len(pts)], precision=33, ppl=ppl)
+    err = np.linalg.norm(b - want, axis=1) / np.linalg.norm(want, axis=1)
+    assert err.max() < 2e-5, err.max()
+
+
 @pytest.mark.parametrize("ppl", [-2, -4])
 def test_field_fp32_small_blocks_batched(ppl):
     """Split-wave kernel (32- or 16-point blocks) on two designs with uneven point and segment counts."""

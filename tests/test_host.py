@@ -148,3 +148,17 @@ def test_beamline_builder_errors():
 def test_single_hip_runtime_in_process():
     """The library shares torch's HIP runtime (one libamdhip64 mapped), so torch buffers are valid for it."""
     assert len(N.loaded_runtimes()) == 1
+
+
+def test_population_adapter_host_parts():
+    """geatpy is optional: the problem description and the design builder work without it."""
+    from cctpy_amd import optim
+    assert len(optim.LB) == len(optim.UB) == len(optim.VAR_TYPES) == optim.DIM == 27
+    assert all(lo <= hi for lo, hi in zip(optim.LB, optim.UB))
+    bl = optim.create_gantry_beamline(part_per_winding=60)
+    assert bl.get_length() > 7.0 and len(bl.magnets) > 10
+    try:
+        import geatpy  # noqa: F401
+    except ImportError:
+        with pytest.raises(ImportError):
+            optim.geatpy_problem()
