@@ -219,9 +219,12 @@ __device__ inline void store3(double* __restrict__ a, int64_t ptot, int64_t g, c
     a[2 * ptot + g] = v[2];
 }
 
+// A particle that has finished its steps takes no part in the backward pass: zero weight at a
+// parking point away from every segment (pad rows of the segment set sit at +1e3, so the
+// parking point must differ from them, else |r| = 0 there).
 __device__ inline void set_bs(double* __restrict__ bs_pts, double* __restrict__ bs_wts, int64_t ptot, int64_t g,
                               const double p[3], const double w[3], bool active) {
-    const double far[3] = {1.0e3, 1.0e3, 1.0e3}, zero[3] = {0.0, 0.0, 0.0};
+    const double far[3] = {-1.0e3, -1.0e3, -1.0e3}, zero[3] = {0.0, 0.0, 0.0};
     store3(bs_pts, ptot, g, active ? p : far);
     store3(bs_wts, ptot, g, active ? w : zero);
 }
