@@ -299,9 +299,17 @@ def main() -> int:
         log(f"[bench] fp64: {ms64:.1f} ms/pass, {rate64:.4g} particle-steps/s; fp32 position error "
             f"max {err['max']:.3g} um, rms {err['rms']:.3g} um")
 
-    # live timing of the dominant kernel (split-K Biot-Savart, FP32) on the stream it runs on
-    pts = torch.from_numpy(np.ascontiguousarray(out32[:, :3].T)).to(engine.device())
-    avg_ms = bs_launch_timing(plan32, pts, partial32, args.roofline_launches)
+    # live timing of the dominant kernel (split-K Biot-Savart, FP32) on the stream it runs on, at
+    # states recorded along the FP32 track (its cost depends on how close particles get to windings)
+    every = max(1, S.max_steps // 10)
+    rec = torch.empty((S.max_steps // every + 1, 6, S.P), dtype=torch.float64, device=engine.device())
+    S.st.copy_(S.s0)
+    scratch32, _ = model.workspace(plan32)
+    model.track_tensors(plan32, FP32, S.st, S.kq, S.dt, S.ns_t, S.max_steps, scratch32, partial32, record=rec,
+                        record_every=every)
+    per = max(5, args.roofline_launches // rec.shape[0])
+    avg_ms = float(np.mean([bs_launch_timing(plan32, rec[i, :3].contiguous(), partial32, per)
+                            for i in range(rec.shape[0])]))
     flops = float(np.sum(np.asarray(S.npt) * model.design_nseg)) * FLOP_PER_PAIR
     achieved = flops / (avg_ms / 1e3) / 1e12
     kname = kernel_name(plan32.ppl)

@@ -3,8 +3,9 @@
 //
 // Kernel map (one HIP stream, no allocation inside the library):
 //   k_winding_generate  CCT parameters -> FP64 segments in the global frame
-//   k_pack_fp32         FP64 segments  -> FP32 tiles re-centred on a per-tile
-//                                         FP64 origin (+ K/C rows)
+//   k_tile_origins      per 32-segment tile: FP64 origin (mean midpoint) and extent
+//   k_pack_fp32         FP64 segments  -> FP32 tiles re-centred on the tile
+//                                         origin (+ K/C rows)
 //   k_bs_partial_*<N>   split-K Biot-Savart: one workgroup = one point block x
 //                       one chunk of tiles; the 4 waves share the block and
 //                       interleave tiles; segments are wave-uniform (scalar
@@ -67,25 +68,61 @@ __global__ void k_winding_points(const double* __restrict__ params, int64_t npoi
     out[3 * i + 2] = p[2];
 }
 
+// Tile origin c = mean of the tile's current-carrying midpoints (a tile of pads
+// keeps its first midpoint) and tile extent max |m - c|^2 over those rows.
+__global__ void k_tile_origins(const double* __restrict__ seg64, int64_t ntile, double* __restrict__ tile_org) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntile) return;
+    const double* rows = seg64 + t * kTile * 8;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    int n = 0;
+    for (int k = 0; k < kTile; ++k) {
+        const double* r = rows + k * 8;
+        if (r[0] == 0.0 && r[1] == 0.0 && r[2] == 0.0) continue;
+        sx += r[4];
+        sy += r[5];
+        sz += r[6];
+        ++n;
+    }
+    double cx = rows[4], cy = rows[5], cz = rows[6];
+    if (n > 0) {
+        cx = sx / n;
+        cy = sy / n;
+        cz = sz / n;
+    }
+    double ext = 0.0;
+    for (int k = 0; k < kTile; ++k) {
+        const double* r = rows + k * 8;
+        if (r[0] == 0.0 && r[1] == 0.0 && r[2] == 0.0) continue;
+        const double dx = r[4] - cx, dy = r[5] - cy, dz = r[6] - cz;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        ext = d2 > ext ? d2 : ext;
+    }
+    tile_org[4 * t] = cx;
+    tile_org[4 * t + 1] = cy;
+    tile_org[4 * t + 2] = cz;
+    tile_org[4 * t + 3] = ext;
+}
+
 __global__ void k_pack_fp32(const double* __restrict__ seg64, int64_t nseg, float* __restrict__ seg32,
-                            double* __restrict__ tile_org, float* __restrict__ seg32k) {
+                            const double* __restrict__ tile_org, float* __restrict__ seg32k) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nseg) return;
     const int64_t t = i / kTile;
     const double* s = seg64 + i * 8;
-    const double* o = seg64 + t * kTile * 8 + 4;  // tile origin: first midpoint of the tile
+    const double* o = tile_org + 4 * t;
+    const double mx = s[4] - o[0], my = s[5] - o[1], mz = s[6] - o[2];
     float* d = seg32 + i * 8;
     d[0] = (float)s[0];
     d[1] = (float)s[1];
     d[2] = (float)s[2];
     d[3] = 0.0f;
-    d[4] = (float)(s[4] - o[0]);
-    d[5] = (float)(s[5] - o[1]);
-    d[6] = (float)(s[6] - o[2]);
+    d[4] = (float)mx;
+    d[5] = (float)my;
+    d[6] = (float)mz;
     d[7] = 0.0f;
     if (seg32k) {
         // {kl, kl x (m - c), -2 (m - c), |m - c|^2}, each rounded once from FP64
-        const double mx = s[4] - o[0], my = s[5] - o[1], mz = s[6] - o[2];
         float* e = seg32k + i * 12;
         e[0] = (float)s[0];
         e[1] = (float)s[1];
@@ -98,21 +135,6 @@ __global__ void k_pack_fp32(const double* __restrict__ seg64, int64_t nseg, floa
         e[8] = (float)(-2.0 * mz);
         e[9] = (float)(mx * mx + my * my + mz * mz);
         e[10] = e[11] = 0.0f;
-    }
-    if (i % kTile == 0) {
-        // extent of the tile: max |m - c|^2 over its current-carrying rows (pads excluded)
-        double ext = 0.0;
-        for (int k = 0; k < kTile && i + k < nseg; ++k) {
-            const double* r = seg64 + (i + k) * 8;
-            if (r[0] == 0.0 && r[1] == 0.0 && r[2] == 0.0) continue;
-            const double dx = r[4] - o[0], dy = r[5] - o[1], dz = r[6] - o[2];
-            const double d2 = dx * dx + dy * dy + dz * dz;
-            ext = d2 > ext ? d2 : ext;
-        }
-        tile_org[t * 4 + 0] = o[0];
-        tile_org[t * 4 + 1] = o[1];
-        tile_org[t * 4 + 2] = o[2];
-        tile_org[t * 4 + 3] = ext;
     }
 }
 
@@ -179,11 +201,12 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32(const float* __restrict_
 // K x q - C; and |q - m|^2 = (|q|^2 + |m|^2) - 2 q.m needs no difference
 // vector.  13 VALU slots per pair (add, 3 fma, rsq, 2 mul, 6 fma) instead of 17.
 // |q|^2 is formed in FP64 per tile.  Both rewrites cancel when the point is
-// close to the tile (|q| of the order of the tile extent e): for |q| >= 4 e the
-// cancellation factors stay below 2, so a wave that has any lane with
-// |q|^2 < 16 e^2 (a point near or inside the windings) sums that tile in the
+// close to the tile (|q| of the order of the tile extent e, the largest |m - c|
+// with c the mean midpoint): for |q| >= 2.5 e the factors (|q|^2 + e^2) / |r|^2
+// and |q| / |r| stay below 3.3, so a wave that has any lane with
+// |q|^2 < 6.25 e^2 (a point near or inside the windings) sums that tile in the
 // direct form (kl x r) |r|^-3 instead.
-constexpr double kNearTile = 16.0;
+constexpr double kNearTile = 6.25;
 
 __device__ inline void tile_direct_term(const float4& a, const float4& b, const float4& c, float qx, float qy,
                                         float qz, float& bx, float& by, float& bz) {
@@ -657,6 +680,9 @@ int cctamd_pack_fp32(const double* seg64, int64_t nseg, float* seg32, double* ti
     const int threads = 256;
     const int64_t blocks = ceil_div(nseg, threads);
     if (blocks > 0x7fffffff) return CCTAMD_ERANGE;
+    const int64_t ntile = nseg / kTile;
+    hipLaunchKernelGGL(k_tile_origins, dim3((unsigned)ceil_div(ntile, threads)), dim3(threads), 0,
+                       (hipStream_t)stream, seg64, ntile, tile_org);
     hipLaunchKernelGGL(k_pack_fp32, dim3((unsigned)blocks), dim3(threads), 0, (hipStream_t)stream, seg64, nseg,
                        seg32, tile_org, seg32k);
     return hip_status(hipGetLastError());

@@ -70,6 +70,8 @@ int cctamd_winding_points(const double *params, int64_t npoints, double *out, vo
 
 /*
  * FP32 tile packing of a FP64 segment set (nseg a multiple of CCTAMD_TILE).
+ * tile_org[t] = {c, e2}: c = mean of the tile's current-carrying midpoints (FP64),
+ * e2 = max |m - c|^2 over them; seg32 rows are {kl, 0, m - c, 0} in FP32.
  * seg32k (nullable) receives the 12-float rows {kl, kl x (m-c), -2(m-c), |m-c|^2, 0, 0}
  * of precision 33 (see cctamd_bs_field).
  */

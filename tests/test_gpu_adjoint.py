@@ -126,3 +126,35 @@ def test_adjoint_population_current_gradients_vs_fd():
         got.append(cct_grad[c, 0] / cur)
     got, fd = np.array(got), np.array(fd)
     assert np.max(np.abs(got - fd)) / np.abs(fd).max() < 1e-4, (got, fd)
+
+
+def test_adjoint_vs_fp64_cpu_oracle_gradient():
+    """The GPU adjoint (FP32 Biot-Savart, precision 33) against central differences of the FP64
+    CPU oracle (C restatement of the reference's CPU tracking) on the 10-parameter vector."""
+    from oracle import c_oracle as C, cct_oracle as O, workload_builder as WB
+    n, length, step = 60, 1.0, 5e-3
+    beam = W.gaussian_beam(np.random.default_rng(23), 12, 0.02)
+    prob, _ = _problem(12, n, length, step, 33, 20)
+    prob.particles = lambda bl: W.beam_particles(W.Beam(beam, 0.4, 250.0), bl)
+    theta = W.section_theta0()
+    final = prob.forward(theta)
+    rng = np.random.default_rng(4)
+    w = rng.normal(size=final.shape) * np.array([1.0, 1.0, 1.0, 1e-8, 1e-8, 1e-8])
+    grad = prob.backward(w)
+
+    def J_cpu(th):
+        bl = WB.build_beamline(W.section_ops(th, n, 0.0))
+        st, kq, speed = WB.beam_state(beam, 0.4, 250.0, bl)
+        sd = [O.rk4_steps(length, step, s) for s in speed]
+        out = C.track(O.beamline_arrays(bl.magnets), st, kq, np.array([d for _, d in sd]),
+                      np.array([s for s, _ in sd], np.int64))
+        return float(np.sum(w * out))
+
+    fd = np.zeros_like(theta)
+    for k in range(theta.size):
+        h = 1e-6 * max(abs(theta[k]), 1.0)
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd[k] = (J_cpu(tp) - J_cpu(tm)) / (2 * h)
+    assert np.max(np.abs(grad - fd)) / np.abs(fd).max() < 1e-4, (grad, fd)
