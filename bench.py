@@ -116,7 +116,7 @@ def cpu_baseline(wl, n_particles: int, n_steps: int):
 class Setup:
     """A workload resident on the device: designs, particles, step counts."""
 
-    def __init__(self, wl, align_cct: bool = False):
+    def __init__(self, wl, align_cct: bool = False, max_steps: int = 0):
         import numpy as np
         import torch
         from cctpy_amd import engine
@@ -135,6 +135,8 @@ class Setup:
         lengths = [wl.beam_length(i) for i, n in enumerate(npt) for _ in range(n)]
         steps_dt = [engine.rk4_steps(L, wl.footstep, p.speed) for L, p in zip(lengths, ps)]
         self.ns = np.array([s for s, _ in steps_dt], np.int64)
+        if max_steps > 0:  # a prefix of every track (same per-step work)
+            self.ns = np.minimum(self.ns, max_steps)
         dt = np.array([d for _, d in steps_dt])
         self.max_steps = int(self.ns.max())
         t0 = time.perf_counter()
@@ -243,6 +245,7 @@ def main() -> int:
     ap.add_argument("--roofline-only", action="store_true", help="only launch the Biot-Savart kernel (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0, help="plain copy of this many MiB (counter calibration)")
     ap.add_argument("--graph", action="store_true", help="replay each tracking pass as one captured HIP graph")
+    ap.add_argument("--max-steps", type=int, default=0, help="cap every track at this many RK4 steps (0: full)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -256,7 +259,7 @@ def main() -> int:
     from cctpy_amd import engine
     from cctpy_amd import workloads as W
 
-    S = Setup(W.by_name(args.workload))
+    S = Setup(W.by_name(args.workload), max_steps=args.max_steps)
     wl, model = S.wl, S.model
     tuned = {FP32: S.shape(FP32, args.ppl, args.items_per_cu)}
     log(f"[bench] fp32 shape: {tuned[FP32]}")
@@ -368,7 +371,8 @@ def main() -> int:
         "dtype": "f32 Biot-Savart terms; f64 tile sums and RK4 state",
         "data": "synthetic",
         "config": {"workload": wl.name, "designs": len(S.bls), "segments": int(model.nseg), "protons": S.P,
-                   "rk4_steps": S.max_steps, "footstep_mm": wl.footstep * 1e3, "length_m": wl.length,
+                   "rk4_steps": S.max_steps, "steps_capped": args.max_steps > 0,
+                   "footstep_mm": wl.footstep * 1e3, "length_m": wl.length,
                    "parallelism": "single GPU", "ppl": plan32.ppl, "work_items": plan32.nwork, "hip_graph": args.graph,
                    "target_items": tuned[FP32]["target_items"]},
         "pair_evals_per_s": S.pairs_per_pass / (ms32 / 1e3),
