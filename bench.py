@@ -156,10 +156,25 @@ class Setup:
         log(f"[bench] {wl.name}: {len(self.bls)} design(s), {self.model.nseg} segments, {self.P} protons, "
             f"{self.max_steps} RK4 steps; windings generated on the GPU in {self.gen_s:.2f} s")
 
+    def track_samples(self, n: int = 8):
+        """Positions (3, P) sampled along a quick FP32 track with a default shape."""
+        import torch
+        from cctpy_amd import engine
+        plan = self.model.plan(self.npt, 2)
+        scratch, partial = self.model.workspace(plan)
+        every = max(1, self.max_steps // n)
+        rec = torch.empty((self.max_steps // every + 1, 6, self.P), dtype=torch.float64, device=engine.device())
+        self.st.copy_(self.s0)
+        self.model.track_tensors(plan, FP32, self.st, self.kq, self.dt, self.ns_t, self.max_steps, scratch, partial,
+                                 record=rec, record_every=every)
+        return [rec[i, :3].contiguous() for i in range(rec.shape[0])]
+
     def shape(self, prec: int, ppl: int, items_per_cu: int) -> dict:
         from cctpy_amd import autotune, engine
         if ppl == 0:
-            return autotune.tune(self.model, self.npt, prec, self.s0[:3].contiguous())
+            if not hasattr(self, "_samples"):
+                self._samples = self.track_samples()
+            return autotune.tune(self.model, self.npt, prec, self._samples)
         if prec != FP32 and ppl < 0:
             ppl = 2
         return {"ppl": ppl, "target_items": items_per_cu * engine.cu_count() or None}
@@ -201,8 +216,7 @@ class Setup:
         from cctpy_amd.adjoint import AdjointTracker
         from cctpy_amd.beam import BeamGroups
         from cctpy_amd.particles import ParticleFactory
-        ppl = shape["ppl"] if shape["ppl"] > 0 else 1
-        tr = AdjointTracker(self.bls, precision=FP32, ckpt_every=CKPT_EVERY, ppl=ppl)
+        tr = AdjointTracker(self.bls, precision=FP32, ckpt_every=CKPT_EVERY, ppl=shape["ppl"])
         tr.prepare(self.npt, self.kq, self.dt, self.ns_t, self.max_steps)
         ideals, frames = [], []
         for i, (bl, beam) in enumerate(zip(self.bls, self.wl.beams)):

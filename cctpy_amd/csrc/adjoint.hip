@@ -177,6 +177,132 @@ __global__ __launch_bounds__(kWG) void k_bs_backward_f32(
     write_partial<PPL>(ax, ay, az, wave, lane, partial + partial_base(w, PB), w.nchunk);
 }
 
+// Split-wave form of k_bs_backward_f32 for 32- or 16-point blocks: each wave
+// splits into G lane groups that hold the same points and walk different tiles
+// (seg32 rows and tangents staged in LDS, read as broadcasts); the per-segment
+// sums reduce within a group, and the point adjoints across groups at the end.
+template <int G>
+__global__ __launch_bounds__(kWG) void k_bs_backward_f32_grp(
+    const float* __restrict__ seg32, const double* __restrict__ tile_org, const float* __restrict__ seg_tan,
+    const int32_t* __restrict__ tile_cct, const double* __restrict__ cct_coef, const Work* __restrict__ work,
+    const double* __restrict__ pts, const double* __restrict__ wts, int64_t ptot, int64_t ntile_total,
+    double* __restrict__ partial, double* __restrict__ tgrad) {
+    constexpr int PB = kWave / G;
+    constexpr int NS = kWavesPerWG * G;  // tile streams per workgroup
+    constexpr int SROW = kTile * 2;      // float4 per tile of seg32
+    constexpr int TROW = kTile * 4;      // float4 per tile of tangents
+    __shared__ float4 lseg[NS][SROW + 1];
+    __shared__ float4 ltan[NS][TROW + 1];
+    __shared__ double red[kWavesPerWG][3][PB];
+    const Work w = work[blockIdx.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int gl = lane % PB;
+    const int stream = wave * G + lane / PB;
+    const bool has = gl < w.pt_count;
+    const int64_t gp = (int64_t)w.pt_start + (has ? gl : 0);
+    const double px = pts[gp], py = pts[ptot + gp], pz = pts[2 * ptot + gp];
+    const float bx = has ? (float)wts[gp] : 0.0f;  // surplus lanes carry no weight
+    const float by = has ? (float)wts[ptot + gp] : 0.0f;
+    const float bz = has ? (float)wts[2 * ptot + gp] : 0.0f;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    const int tend = w.tile_start + w.tile_count;
+    const float4* s4 = reinterpret_cast<const float4*>(seg32);
+    const float4* t4 = reinterpret_cast<const float4*>(seg_tan);
+    const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int t0 = w.tile_start; t0 < tend; t0 += NS) {
+        for (int i = threadIdx.x; i < NS * SROW; i += kWG) {
+            const int s = i / SROW, e = i % SROW;
+            lseg[s][e] = t0 + s < tend ? s4[(int64_t)(t0 + s) * SROW + e] : zero4;
+        }
+        for (int i = threadIdx.x; i < NS * TROW; i += kWG) {
+            const int s = i / TROW, e = i % TROW;
+            ltan[s][e] = t0 + s < tend ? t4[(int64_t)(t0 + s) * TROW + e] : zero4;
+        }
+        __syncthreads();
+        const int t = t0 + stream;
+        if (t < tend) {  // uniform within a lane group
+            const int cct = tile_cct[t];
+            const float coef = cct >= 0 ? (float)cct_coef[cct] : 0.0f;
+            const float qx = (float)(px - tile_org[4 * t]);
+            const float qy = (float)(py - tile_org[4 * t + 1]);
+            const float qz = (float)(pz - tile_org[4 * t + 2]);
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            float cur = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+            const float4* sr = lseg[stream];
+            const float4* tr = ltan[stream];
+#pragma unroll 2
+            for (int k = 0; k < kTile; ++k) {
+                const float4 kl = sr[2 * k];
+                const float4 m = sr[2 * k + 1];
+                const float rx = qx - m.x, ry = qy - m.y, rz = qz - m.z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float w3 = inv * inv * inv;
+                const float tx = kl.y * rz - kl.z * ry, ty = kl.z * rx - kl.x * rz, tz = kl.x * ry - kl.y * rx;
+                const float sdot = bx * tx + by * ty + bz * tz;
+                const float a3 = 3.0f * sdot * w3 * (inv * inv);
+                const float ux = (by * kl.z - bz * kl.y) * w3 - rx * a3;
+                const float uy = (bz * kl.x - bx * kl.z) * w3 - ry * a3;
+                const float uz = (bx * kl.y - by * kl.x) * w3 - rz * a3;
+                sx += ux;
+                sy += uy;
+                sz += uz;
+                const float gkx = (ry * bz - rz * by) * w3, gky = (rz * bx - rx * bz) * w3,
+                            gkz = (rx * by - ry * bx) * w3;
+                cur += sdot * w3;
+                // d kl/dc_j = coef (T' s'_j - T s_j), d m/dc_j = (T' s'_j + T s_j)/2, dJ/dm = -u
+                const float4 ta = tr[4 * k], tb = tr[4 * k + 1], tc = tr[4 * k + 2], td = tr[4 * k + 3];
+                const float Ax = coef * gkx - 0.5f * ux, Ay = coef * gky - 0.5f * uy, Az = coef * gkz - 0.5f * uz;
+                const float Bx = -0.5f * ux - coef * gkx, By = -0.5f * uy - coef * gky, Bz = -0.5f * uz - coef * gkz;
+                const float alpha = Ax * tc.x + Ay * tc.y + Az * tc.z;
+                const float beta = Bx * ta.x + By * ta.y + Bz * ta.z;
+                g0 += tc.w * alpha + ta.w * beta;
+                g1 += td.x * alpha + tb.x * beta;
+                g2 += td.y * alpha + tb.y * beta;
+                g3 += td.z * alpha + tb.z * beta;
+            }
+            ax += (double)sx;
+            ay += (double)sy;
+            az += (double)sz;
+#pragma unroll
+            for (int off = PB / 2; off > 0; off >>= 1) {  // sum over the group's points
+                cur += __shfl_xor(cur, off);
+                g0 += __shfl_xor(g0, off);
+                g1 += __shfl_xor(g1, off);
+                g2 += __shfl_xor(g2, off);
+                g3 += __shfl_xor(g3, off);
+            }
+            if (gl == 0 && cct >= 0) {
+                double* tg = tgrad + ((int64_t)w.block * ntile_total + t) * kTileGrad;
+                tg[0] += (double)cur;
+                tg[1] += (double)g0;
+                tg[2] += (double)g1;
+                tg[3] += (double)g2;
+                tg[4] += (double)g3;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int off = PB; off < kWave; off <<= 1) {  // add the lane groups (fixed order)
+        ax += __shfl_xor(ax, off);
+        ay += __shfl_xor(ay, off);
+        az += __shfl_xor(az, off);
+    }
+    if (lane < PB) {
+        red[wave][0][lane] = ax;
+        red[wave][1][lane] = ay;
+        red[wave][2][lane] = az;
+    }
+    __syncthreads();
+    double* out = partial + partial_base(w, PB);
+    for (int k = threadIdx.x; k < 3 * PB; k += kWG) {
+        const int c = k / PB, i = k % PB;
+        out[(int64_t)(i * 3 + c) * w.nchunk] = ((red[0][c][i] + red[1][c][i]) + red[2][c][i]) + red[3][c][i];
+    }
+}
+
 // ------------------------------------------------------ stage adjoints
 // Per-particle adjoint rows (adj[r][P]): 0-2 pbar, 3-5 vbar (end of step),
 // 6-8 sum of P_k-bar, 9-11 sum of V_k-bar, 12-14 V_k-bar of the stage just done.
@@ -342,6 +468,16 @@ hipError_t launch_bwd_any(int ppl, const float* seg32, const double* tile_org, c
                           const int32_t* tile_cct, const double* cct_coef, const Work* work, int64_t nwork,
                           const double* pts, const double* wts, int64_t ptot, int64_t ntile, double* partial,
                           double* tgrad, hipStream_t st) {
+    if (ppl < 0) {
+        if (nwork == 0) return hipSuccess;
+        if (ppl == -2)
+            hipLaunchKernelGGL(k_bs_backward_f32_grp<2>, dim3((unsigned)nwork), dim3(kWG), 0, st, seg32, tile_org,
+                               seg_tan, tile_cct, cct_coef, work, pts, wts, ptot, ntile, partial, tgrad);
+        else
+            hipLaunchKernelGGL(k_bs_backward_f32_grp<4>, dim3((unsigned)nwork), dim3(kWG), 0, st, seg32, tile_org,
+                               seg_tan, tile_cct, cct_coef, work, pts, wts, ptot, ntile, partial, tgrad);
+        return hipGetLastError();
+    }
     switch (ppl) {
         case 1: return launch_bwd<1>(seg32, tile_org, seg_tan, tile_cct, cct_coef, work, nwork, pts, wts, ptot, ntile,
                                      partial, tgrad, st);
@@ -380,7 +516,7 @@ int cctamd_track_rk4_adjoint(int precision, int ppl, const void* seg, const doub
                              const double* dt, const int64_t* nsteps, int64_t max_steps, double* adj, double* tgrad,
                              double* qgrad, double* state_ws, double* scratch, double* partial, double* tape,
                              double* adj_ws, double* bs_pts, double* bs_wts, void* stream) {
-    if (!valid_common(precision, ppl, seg, tile_org, nwork, ndesign) || ppl < 0) return CCTAMD_EINVAL;
+    if (!valid_common(precision, ppl, seg, tile_org, nwork, ndesign)) return CCTAMD_EINVAL;
     if (nparticles < 0 || max_steps < 0 || ckpt_every <= 0 || ntile < 0) return CCTAMD_EINVAL;
     if (nparticles == 0 || max_steps == 0) return CCTAMD_OK;
     if (!checkpoints || !kq || !dt || !nsteps || !adj || !tgrad || !qgrad || !state_ws || !scratch || !tape ||
