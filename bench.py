@@ -50,7 +50,9 @@ def log(*a):
 
 
 def kernel_name(ppl: int) -> str:
-    return f"k_bs_partial_f32k<{ppl}>" if ppl > 0 else f"k_bs_partial_f32k_grp<{-ppl}>"
+    """Name of the FP32 tracking kernel a points-per-lane code launches (32/16-point blocks:
+    4/8 lane groups with 2 points per lane)."""
+    return f"k_bs_partial_f32k<{ppl}>" if ppl > 0 else f"k_bs_partial_f32k_grp<{2 * -ppl}, 2>"
 
 
 def measured_traffic(workload: str, kernel: str, nwork: int):

@@ -297,18 +297,23 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k(const float* __restrict
 }
 
 // Small point blocks (32 or 16 points): each wave is split into G lane groups
-// that hold the same points and walk different tiles, so every lane works.
+// that hold the same points (PL per lane) and walk different tiles, so every
+// lane works.  Each wave reads 3 ds_read_b128 per segment whatever PL is, and
+// four waves share the CU's LDS, so two points per lane (4 groups for 32-point
+// blocks, 8 for 16-point blocks) halve the LDS reads per pair of one point per
+// lane; four points per lane cost more in registers (occupancy) than they save.
 // The workgroup stages 4G tiles of K/C rows in LDS per round; a lane reads
-// the segments of its group's tile with LDS broadcasts (2 or 4 addresses per
-// wave), which issue outside the VALU.  Missing tiles of the last round are
+// the segments of its group's tile with LDS broadcasts (G addresses per wave),
+// which issue outside the VALU.  Missing tiles of the last round are
 // zero-current segments far away.
-template <int G>
+template <int G, int PL>
 __global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __restrict__ seg32k,
                                                               const double* __restrict__ tile_org,
                                                               const Work* __restrict__ work,
                                                               const double* __restrict__ pts, int64_t ptot,
                                                               double* __restrict__ partial) {
-    constexpr int PB = kWave / G;
+    constexpr int LG = kWave / G;         // lanes per group
+    constexpr int PB = LG * PL;           // points per block: PL per lane
     constexpr int NS = kWavesPerWG * G;   // tile streams per workgroup
     constexpr int ROW = kTile * 3;        // float4 per tile
     __shared__ float4 lds[NS][ROW + 1];   // +1: the streams start in different banks
@@ -316,13 +321,19 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __rest
     const Work w = work[blockIdx.x];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x >> 6;
-    const int grp = lane / PB;
-    const int stream = wave * G + grp;
-    int idx = lane % PB;
-    idx = idx < w.pt_count ? idx : 0;
-    const int64_t gp = (int64_t)w.pt_start + idx;
-    const double px = pts[gp], py = pts[ptot + gp], pz = pts[2 * ptot + gp];
-    double ax = 0.0, ay = 0.0, az = 0.0;
+    const int gl = lane % LG;
+    const int stream = wave * G + lane / LG;
+    double px[PL], py[PL], pz[PL], ax[PL], ay[PL], az[PL];
+#pragma unroll
+    for (int j = 0; j < PL; ++j) {
+        int idx = gl + LG * j;
+        idx = idx < w.pt_count ? idx : 0;  // surplus lanes duplicate point 0, results discarded
+        const int64_t gp = (int64_t)w.pt_start + idx;
+        px[j] = pts[gp];
+        py[j] = pts[ptot + gp];
+        pz[j] = pts[2 * ptot + gp];
+        ax[j] = ay[j] = az[j] = 0.0;
+    }
     const int tend = w.tile_start + w.tile_count;
     const float4* src = reinterpret_cast<const float4*>(seg32k);
     // missing tiles: zero current, |m|^2 = 1e6 keeps |q - m|^2 > 0
@@ -337,52 +348,80 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __rest
         __syncthreads();
         const bool real = t0 + stream < tend;
         const int t = real ? t0 + stream : w.tile_start;
-        const double dx = px - tile_org[4 * t], dy = py - tile_org[4 * t + 1], dz = pz - tile_org[4 * t + 2];
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        const float qx = (float)dx, qy = (float)dy, qz = (float)dz;
-        const float qq = (float)d2;
-        float kx = 0.0f, ky = 0.0f, kz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        const double ox = tile_org[4 * t], oy = tile_org[4 * t + 1], oz = tile_org[4 * t + 2];
+        const double near2 = kNearTile * tile_org[4 * t + 3];
+        float qx[PL], qy[PL], qz[PL], qq[PL], kx[PL], ky[PL], kz[PL], cx[PL], cy[PL], cz[PL];
+        bool near = false;
+#pragma unroll
+        for (int j = 0; j < PL; ++j) {
+            const double dx = px[j] - ox, dy = py[j] - oy, dz = pz[j] - oz;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            near = near || (real && d2 < near2);
+            qx[j] = (float)dx;
+            qy[j] = (float)dy;
+            qz[j] = (float)dz;
+            qq[j] = (float)d2;
+            kx[j] = ky[j] = kz[j] = cx[j] = cy[j] = cz[j] = 0.0f;
+        }
         const float4* r4 = lds[stream];
-        if (__any(real && d2 < kNearTile * tile_org[4 * t + 3])) {  // direct form for this round
+        if (__any(near)) {  // direct form for this round
 #pragma unroll 1
-            for (int k = 0; k < kTile; ++k)
-                tile_direct_term(r4[3 * k], r4[3 * k + 1], r4[3 * k + 2], qx, qy, qz, kx, ky, kz);
-            ax += (double)kx;
-            ay += (double)ky;
-            az += (double)kz;
+            for (int k = 0; k < kTile; ++k) {
+                const float4 a = r4[3 * k], b = r4[3 * k + 1], c = r4[3 * k + 2];
+#pragma unroll
+                for (int j = 0; j < PL; ++j) tile_direct_term(a, b, c, qx[j], qy[j], qz[j], kx[j], ky[j], kz[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < PL; ++j) {
+                ax[j] += (double)kx[j];
+                ay[j] += (double)ky[j];
+                az[j] += (double)kz[j];
+            }
             __syncthreads();
             continue;
         }
-#pragma unroll 8
+#pragma unroll 4
         for (int k = 0; k < kTile; ++k) {
             const float4 a = r4[3 * k];
             const float4 b = r4[3 * k + 1];
             const float4 c = r4[3 * k + 2];
-            const float r2 = __builtin_fmaf(qz, c.x, __builtin_fmaf(qy, b.w, __builtin_fmaf(qx, b.z, qq + c.y)));
-            const float inv = __builtin_amdgcn_rsqf(r2);
-            const float w3 = inv * inv * inv;
-            kx += a.x * w3;
-            ky += a.y * w3;
-            kz += a.z * w3;
-            cx += a.w * w3;
-            cy += b.x * w3;
-            cz += b.y * w3;
+#pragma unroll
+            for (int j = 0; j < PL; ++j) {
+                const float r2 = __builtin_fmaf(qz[j], c.x, __builtin_fmaf(qy[j], b.w, __builtin_fmaf(qx[j], b.z, qq[j] + c.y)));
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float w3 = inv * inv * inv;
+                kx[j] += a.x * w3;
+                ky[j] += a.y * w3;
+                kz[j] += a.z * w3;
+                cx[j] += a.w * w3;
+                cy[j] += b.x * w3;
+                cz[j] += b.y * w3;
+            }
         }
-        ax += (double)((ky * qz - kz * qy) - cx);
-        ay += (double)((kz * qx - kx * qz) - cy);
-        az += (double)((kx * qy - ky * qx) - cz);
+#pragma unroll
+        for (int j = 0; j < PL; ++j) {
+            ax[j] += (double)((ky[j] * qz[j] - kz[j] * qy[j]) - cx[j]);
+            ay[j] += (double)((kz[j] * qx[j] - kx[j] * qz[j]) - cy[j]);
+            az[j] += (double)((kx[j] * qy[j] - ky[j] * qx[j]) - cz[j]);
+        }
         __syncthreads();
     }
 #pragma unroll
-    for (int off = PB; off < kWave; off <<= 1) {  // add the lane groups (fixed order)
-        ax += __shfl_xor(ax, off);
-        ay += __shfl_xor(ay, off);
-        az += __shfl_xor(az, off);
+    for (int j = 0; j < PL; ++j) {
+#pragma unroll
+        for (int off = LG; off < kWave; off <<= 1) {  // add the lane groups (fixed order)
+            ax[j] += __shfl_xor(ax[j], off);
+            ay[j] += __shfl_xor(ay[j], off);
+            az[j] += __shfl_xor(az[j], off);
+        }
     }
-    if (grp == 0) {
-        red[wave][0][lane] = ax;
-        red[wave][1][lane] = ay;
-        red[wave][2][lane] = az;
+    if (lane < LG) {
+#pragma unroll
+        for (int j = 0; j < PL; ++j) {
+            red[wave][0][lane + LG * j] = ax[j];
+            red[wave][1][lane + LG * j] = ay[j];
+            red[wave][2][lane + LG * j] = az[j];
+        }
     }
     __syncthreads();
     double* out = partial + partial_base(w, PB);
@@ -591,10 +630,10 @@ hipError_t launch_bs_any(int precision, int ppl, const void* seg, const double* 
         if (nwork == 0) return hipSuccess;
         if (precision != 33) return hipErrorInvalidValue;
         if (ppl == -2)
-            hipLaunchKernelGGL(k_bs_partial_f32k_grp<2>, dim3((unsigned)nwork), dim3(kWG), 0, st,
+            hipLaunchKernelGGL((k_bs_partial_f32k_grp<4, 2>), dim3((unsigned)nwork), dim3(kWG), 0, st,
                                (const float*)seg, tile_org, work, pts, ptot, partial);
         else
-            hipLaunchKernelGGL(k_bs_partial_f32k_grp<4>, dim3((unsigned)nwork), dim3(kWG), 0, st,
+            hipLaunchKernelGGL((k_bs_partial_f32k_grp<8, 2>), dim3((unsigned)nwork), dim3(kWG), 0, st,
                                (const float*)seg, tile_org, work, pts, ptot, partial);
         return hipGetLastError();
     }
