@@ -212,13 +212,14 @@ class Setup:
         ms = self.timed(one_pass, passes, warmup) / passes
         return ms, self.st.cpu().numpy().T.copy(), plan, partial
 
-    def forward_adjoint(self, shape: dict, passes: int, warmup: int):
-        """ms per FP32 forward (with checkpoints) + objective + adjoint pass, and the split."""
+    def forward_adjoint(self, shape: dict, passes: int, warmup: int, ckpt_every: int = CKPT_EVERY):
+        """ms per FP32 forward (with checkpoints every ckpt_every steps, or the full tape with 0)
+        + objective + adjoint pass, and the split."""
         import torch
         from cctpy_amd.adjoint import AdjointTracker
         from cctpy_amd.beam import BeamGroups
         from cctpy_amd.particles import ParticleFactory
-        tr = AdjointTracker(self.bls, precision=FP32, ckpt_every=CKPT_EVERY, ppl=shape["ppl"])
+        tr = AdjointTracker(self.bls, precision=FP32, ckpt_every=ckpt_every, ppl=shape["ppl"])
         tr.prepare(self.npt, self.kq, self.dt, self.ns_t, self.max_steps)
         ideals, frames = [], []
         for i, (bl, beam) in enumerate(zip(self.bls, self.wl.beams)):
@@ -241,7 +242,7 @@ class Setup:
         J = BeamGroups.objective(out["stats"])
         return {"ms_per_pass": ms, "forward_ms": fwd, "adjoint_ms": ms - fwd,
                 "particle_steps_per_s": self.particle_steps / (ms / 1e3), "ppl": tr.plan.ppl,
-                "checkpoint_every": CKPT_EVERY, "objective_mean": float(J.mean()),
+                "checkpoint_every": ckpt_every, "objective_mean": float(J.mean()),
                 "grad_finite": bool(torch.isfinite(tr.tgrad).all().item())}
 
 
@@ -262,6 +263,9 @@ def main() -> int:
     ap.add_argument("--copy-mib", type=int, default=0, help="plain copy of this many MiB (counter calibration)")
     ap.add_argument("--graph", action="store_true", help="replay each tracking pass as one captured HIP graph")
     ap.add_argument("--max-steps", type=int, default=0, help="cap every track at this many RK4 steps (0: full)")
+    ap.add_argument("--ckpt-every", type=int, default=CKPT_EVERY, help="adjoint mode: checkpoint interval (0: full tape)")
+    ap.add_argument("--mode", choices=("forward", "adjoint"), default="forward",
+                    help="adjoint: time only FP32 forward + objective + adjoint passes of the workload (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -274,6 +278,14 @@ def main() -> int:
 
     from cctpy_amd import engine
     from cctpy_amd import workloads as W
+
+    if args.mode == "adjoint":
+        A_ = Setup(W.by_name(args.workload), align_cct=True, max_steps=args.max_steps)
+        fa = A_.forward_adjoint(A_.shape(FP32, args.ppl, args.items_per_cu), args.steps, args.warmup,
+                                ckpt_every=args.ckpt_every)
+        print(json.dumps(dict(workload=A_.wl.name, protons=A_.P, segments=int(A_.model.nseg),
+                              rk4_steps=A_.max_steps, **fa)), flush=True)
+        return 0
 
     S = Setup(W.by_name(args.workload), max_steps=args.max_steps)
     wl, model = S.wl, S.model
@@ -350,20 +362,26 @@ def main() -> int:
         pm64, pout64, _, _ = P_.forward(64, sh64, 1, 1, True)
         d = np.linalg.norm(pout32[:, :3] - pout64[:, :3], axis=1)
         fa = P_.forward_adjoint(sh, 1, 1)
+        fa0 = P_.forward_adjoint(sh, 1, 1, ckpt_every=0)
         extras["population"] = {
             "designs": len(P_.bls), "segments": int(P_.model.nseg), "protons": P_.P, "rk4_steps_max": P_.max_steps,
             "fp32_particle_steps_per_s": P_.particle_steps / (pm32 / 1e3), "fp32_ms_per_pass": pm32,
             "fp32_shape": sh, "fp64_particle_steps_per_s": P_.particle_steps / (pm64 / 1e3), "fp64_ms_per_pass": pm64,
             "fp32_vs_fp64": pm64 / pm32, "position_error_um_vs_fp64": {"max": float(d.max() * 1e6),
                                                                        "rms": float(np.sqrt(np.mean(d ** 2)) * 1e6)},
-            "hip_graph": True, "forward_adjoint": fa,
-            "forward_adjoint_vs_fp64_forward": fa["particle_steps_per_s"] / (P_.particle_steps / (pm64 / 1e3))}
+            "hip_graph": True, "forward_adjoint": fa, "forward_adjoint_full_tape": fa0,
+            "forward_adjoint_vs_fp64_forward": fa["particle_steps_per_s"] / (P_.particle_steps / (pm64 / 1e3)),
+            "forward_adjoint_full_tape_vs_fp64_forward":
+                fa0["particle_steps_per_s"] / (P_.particle_steps / (pm64 / 1e3))}
         log(f"[bench] population: fp32 {pm32:.1f} ms, fp64 {pm64:.1f} ms, fwd+adj {fa['ms_per_pass']:.1f} ms, "
             f"error max {d.max() * 1e6:.3g} um")
         del P_
         A_ = Setup(W.by_name("adjoint_section"), align_cct=True)
-        fa = A_.forward_adjoint(A_.shape(FP32, 0, 0), 1, 1)
-        extras["adjoint_section"] = dict(protons=A_.P, segments=int(A_.model.nseg), rk4_steps=A_.max_steps, **fa)
+        sh = A_.shape(FP32, 0, 0)
+        fa = A_.forward_adjoint(sh, 1, 1)
+        fa0 = A_.forward_adjoint(sh, 1, 1, ckpt_every=0)
+        extras["adjoint_section"] = dict(protons=A_.P, segments=int(A_.model.nseg), rk4_steps=A_.max_steps, **fa,
+                                         full_tape=fa0)
         log(f"[bench] adjoint_section: fwd+adj {fa['ms_per_pass']:.1f} ms ({fa['forward_ms']:.1f} forward)")
         del A_
 

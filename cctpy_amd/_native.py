@@ -56,6 +56,9 @@ def _load() -> ctypes.CDLL:
     lib.cctamd_track_rk4_adjoint.argtypes = [INT, INT, P, P, P, P, P, P, I64, P, I64, P, P, P, I64, P, P, P,
                                              P, I64, I64, P, P, P, I64, P, P, P, P, P, P, P, P, P, P, P]
     lib.cctamd_track_rk4_adjoint.restype = INT
+    lib.cctamd_track_rk4_taped.argtypes = [INT, INT, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P, I64, P,
+                                           P, P, P]
+    lib.cctamd_track_rk4_taped.restype = INT
     lib.cctamd_phase_space.argtypes = [P, I64, P, P, P, I64, P, P, I64, P, P, P]
     lib.cctamd_phase_space.restype = INT
     lib.cctamd_beam_objective_seed.argtypes = [P, I64, P, P, I64, P, P, I64, P, P, P]
@@ -81,5 +84,6 @@ def check(code: int, what: str) -> None:
 EXPORTED = (
     "cctamd_version", "cctamd_error_string", "cctamd_winding_generate", "cctamd_winding_points",
     "cctamd_pack_fp32", "cctamd_plan", "cctamd_bs_partial", "cctamd_bs_field", "cctamd_track_rk4",
-    "cctamd_winding_tangents", "cctamd_track_rk4_adjoint", "cctamd_phase_space", "cctamd_beam_objective_seed",
+    "cctamd_winding_tangents", "cctamd_track_rk4_taped", "cctamd_track_rk4_adjoint", "cctamd_phase_space",
+    "cctamd_beam_objective_seed",
 )

@@ -28,6 +28,11 @@ class AdjointTracker:
     """Forward tracking with checkpoints and its adjoint for one set of designs."""
 
     def __init__(self, magnets: Sequence[Magnet], precision: int = 33, ckpt_every: int = 50, ppl: int = 2):
+        """``ckpt_every`` K > 0: checkpoints every K steps and recompute in the reverse pass;
+        K = 0: the forward keeps the tape of every step (288 B per particle-step of HBM) and the
+        reverse pass recomputes nothing."""
+        if ckpt_every < 0:
+            raise ValueError("ckpt_every must be >= 0")
         specs = []
         for m in magnets:
             parts = device_parts(m)
@@ -50,8 +55,11 @@ class AdjointTracker:
         self.P = P = plan.npt
         self.kq, self.dt, self.ns = kq, dt, nsteps
         self.max_steps = int(max_steps)
-        nck = -(-self.max_steps // self.ckpt_every)
-        self.ckpt = torch.zeros((nck + 1, 6, P), dtype=torch.float64, device=dev)
+        if self.ckpt_every > 0:
+            nck = -(-self.max_steps // self.ckpt_every)
+            self.ckpt = torch.zeros((nck + 1, 6, P), dtype=torch.float64, device=dev)
+        else:  # full tape, no recompute: 288 B per particle-step
+            self.ckpt = None
         self.scratch, self.partial = m.workspace(plan)
         self.ntile = max(m.nseg_padded // N.TILE, 1)
         pb = engine.points_per_block(plan.ppl)
@@ -60,15 +68,26 @@ class AdjointTracker:
         self.nq = nq = int(m.design_qs_off_host[-1])
         self.qgrad = torch.zeros((max(2 * nq, 1), P), dtype=torch.float64, device=dev)
         self.state_ws = torch.empty((6, P), dtype=torch.float64, device=dev)
-        self.tape = torch.zeros((self.ckpt_every, 36, P), dtype=torch.float64, device=dev)
+        self.tape = torch.zeros((self.ckpt_every if self.ckpt_every > 0 else max(self.max_steps, 1), 36, P),
+                                dtype=torch.float64, device=dev)
         self.adj_ws = torch.zeros((15, P), dtype=torch.float64, device=dev)
         self.bs_pts = torch.zeros((3, P), dtype=torch.float64, device=dev)
         self.bs_wts = torch.zeros((3, P), dtype=torch.float64, device=dev)
 
     def forward_tensors(self, state: torch.Tensor) -> None:
-        """In-place tracking of device state (6, P), keeping checkpoints every K steps."""
-        self.model.track_tensors(self.plan, self.precision, state, self.kq, self.dt, self.ns, self.max_steps,
-                                 self.scratch, self.partial, record=self.ckpt, record_every=self.ckpt_every)
+        """In-place tracking of device state (6, P), keeping checkpoints every K steps (or, with
+        K = 0, the tape of every step)."""
+        if self.ckpt_every > 0:
+            self.model.track_tensors(self.plan, self.precision, state, self.kq, self.dt, self.ns, self.max_steps,
+                                     self.scratch, self.partial, record=self.ckpt, record_every=self.ckpt_every)
+            return
+        m, plan = self.model, self.plan
+        seg, org = m._seg(self.precision)
+        N.check(N.lib.cctamd_track_rk4_taped(
+            self.precision, plan.ppl, _p(seg), _p(org), _p(plan.work), plan.nwork, _p(plan.wbase), _p(plan.nchunk),
+            _p(plan.design_pt_off), m.ndesign, _p(m.qs), _p(m.design_qs_off), _p(m.bconst), _p(state), self.P,
+            _p(self.kq), _p(self.dt), _p(self.ns), self.max_steps, _p(self.scratch), _p(self.partial), _p(self.tape),
+            _stream()), "cctamd_track_rk4_taped")
 
     def backward_tensors(self, adj: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
         """In-place reverse pass: adj (6, P) dJ/d(final state) -> dJ/d(initial state).

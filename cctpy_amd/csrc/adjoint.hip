@@ -507,6 +507,26 @@ int cctamd_winding_tangents(const double* params, const int64_t* count_off, cons
     return hip_status(hipGetLastError());
 }
 
+int cctamd_track_rk4_taped(int precision, int ppl, const void* seg, const double* tile_org, const int32_t* work,
+                           int64_t nwork, const int64_t* design_wbase, const int32_t* design_nchunk,
+                           const int64_t* design_pt_off, int64_t ndesign, const double* qs,
+                           const int64_t* design_qs_off, const double* bconst, double* state, int64_t nparticles,
+                           const double* kq, const double* dt, const int64_t* nsteps, int64_t max_steps,
+                           double* scratch, double* partial, double* tape, void* stream) {
+    if (!valid_common(precision, ppl, seg, tile_org, nwork, ndesign)) return CCTAMD_EINVAL;
+    if (nparticles < 0 || max_steps < 0) return CCTAMD_EINVAL;
+    if (nparticles == 0 || max_steps == 0) return CCTAMD_OK;
+    if (!state || !kq || !dt || !nsteps || !scratch || !tape || !design_wbase || !design_nchunk || !design_pt_off ||
+        !design_qs_off || !bconst)
+        return CCTAMD_EINVAL;
+    if (nwork > 0 && (!work || !partial)) return CCTAMD_EINVAL;
+    if (ceil_div(nparticles, kWavesPerWG) > 0x7fffffff) return CCTAMD_ERANGE;
+    const DeviceSet ds{precision, ppl, seg, tile_org, (const Work*)work, nwork, design_wbase, design_nchunk,
+                       design_pt_off, ndesign, qs, design_qs_off, bconst};
+    return hip_status(launch_rk4_steps(ds, state, nparticles, kq, dt, nsteps, 0, max_steps, scratch, partial, nullptr,
+                                       1, tape, (hipStream_t)stream));
+}
+
 int cctamd_track_rk4_adjoint(int precision, int ppl, const void* seg, const double* tile_org, const float* seg32,
                              const float* seg_tan, const int32_t* tile_cct, const double* cct_coef, int64_t ntile,
                              const int32_t* work, int64_t nwork, const int64_t* design_wbase,
@@ -517,9 +537,10 @@ int cctamd_track_rk4_adjoint(int precision, int ppl, const void* seg, const doub
                              double* qgrad, double* state_ws, double* scratch, double* partial, double* tape,
                              double* adj_ws, double* bs_pts, double* bs_wts, void* stream) {
     if (!valid_common(precision, ppl, seg, tile_org, nwork, ndesign)) return CCTAMD_EINVAL;
-    if (nparticles < 0 || max_steps < 0 || ckpt_every <= 0 || ntile < 0) return CCTAMD_EINVAL;
+    if (nparticles < 0 || max_steps < 0 || ckpt_every < 0 || ntile < 0) return CCTAMD_EINVAL;
     if (nparticles == 0 || max_steps == 0) return CCTAMD_OK;
-    if (!checkpoints || !kq || !dt || !nsteps || !adj || !tgrad || !qgrad || !state_ws || !scratch || !tape ||
+    const bool full_tape = ckpt_every == 0;  // tape of every step from cctamd_track_rk4_taped: no recompute
+    if ((!full_tape && !checkpoints) || !kq || !dt || !nsteps || !adj || !tgrad || !qgrad || !state_ws || !scratch || !tape ||
         !adj_ws || !bs_pts || !bs_wts || !design_wbase || !design_nchunk || !design_pt_off || !design_qs_off ||
         !bconst)
         return CCTAMD_EINVAL;
@@ -535,14 +556,17 @@ int cctamd_track_rk4_adjoint(int precision, int ppl, const void* seg, const doub
     hipError_t e;
     // adj rows 0-5 hold the seed; copy into the working rows
     hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)ceil_div(6 * P, 256)), dim3(256), 0, st, adj, adj_ws, 6 * P);
-    const int64_t nck = ceil_div(max_steps, ckpt_every);
+    const int64_t interval = full_tape ? max_steps : ckpt_every;
+    const int64_t nck = ceil_div(max_steps, interval);
     for (int64_t c = nck - 1; c >= 0; --c) {
-        const int64_t s0 = c * ckpt_every;
-        const int64_t s1 = s0 + ckpt_every < max_steps ? s0 + ckpt_every : max_steps;
-        hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)ceil_div(6 * P, 256)), dim3(256), 0, st,
-                           checkpoints + c * 6 * P, state_ws, 6 * P);
-        e = launch_rk4_steps(ds, state_ws, P, kq, dt, nsteps, s0, s1, scratch, partial, nullptr, 1, tape, st);
-        if (e != hipSuccess) return (int)e;
+        const int64_t s0 = c * interval;
+        const int64_t s1 = s0 + interval < max_steps ? s0 + interval : max_steps;
+        if (!full_tape) {  // recompute this interval from its checkpoint, recording the tape
+            hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)ceil_div(6 * P, 256)), dim3(256), 0, st,
+                               checkpoints + c * 6 * P, state_ws, 6 * P);
+            e = launch_rk4_steps(ds, state_ws, P, kq, dt, nsteps, s0, s1, scratch, partial, nullptr, 1, tape, st);
+            if (e != hipSuccess) return (int)e;
+        }
         for (int64_t step = s1 - 1; step >= s0; --step) {
             const double* tp = tape + (step - s0) * 36 * P;
             hipLaunchKernelGGL(k_adj_begin, dim3((unsigned)eblk), dim3(256), 0, st, tp, adj_ws, kq, dt, nsteps, step,

@@ -156,6 +156,21 @@ int cctamd_winding_tangents(const double *params, const int64_t *count_off, cons
                             int64_t ncct, int64_t nseg_total, float *seg_tan, void *stream);
 
 /*
+ * cctamd_track_rk4 (record_every = 1, no record) that also keeps the tape of every
+ * step for cctamd_track_rk4_adjoint with ckpt_every = 0 (no counterpart in the
+ * reference): tape double[max_steps][36][P] = per step and RK4 stage the stage
+ * point, velocity and field rows.  Trades memory (288 B per particle-step) for the
+ * recompute of the checkpointed mode.
+ */
+int cctamd_track_rk4_taped(int precision, int ppl, const void *seg, const double *tile_org,
+                           const int32_t *work, int64_t nwork, const int64_t *design_wbase,
+                           const int32_t *design_nchunk, const int64_t *design_pt_off, int64_t ndesign,
+                           const double *qs, const int64_t *design_qs_off, const double *bconst,
+                           double *state, int64_t nparticles, const double *kq, const double *dt,
+                           const int64_t *nsteps, int64_t max_steps, double *scratch, double *partial,
+                           double *tape, void *stream);
+
+/*
  * Reverse-mode derivative of cctamd_track_rk4 (no counterpart in the reference,
  * which has no gradients; it differentiates the tracking that
  * final_code/packages/particles.py:561-632 performs).
@@ -165,13 +180,15 @@ int cctamd_winding_tangents(const double *params, const int64_t *count_off, cons
  *     the backward kernel uses them at every precision, so tile_org (the FP32 tile
  *     origins of cctamd_pack_fp32) is required also when precision is 64;
  *   checkpoints: double[ceil(max_steps/ckpt_every)][6][P], the forward state every
- *     ckpt_every steps (cctamd_track_rk4 with record_every = ckpt_every);
+ *     ckpt_every steps (cctamd_track_rk4 with record_every = ckpt_every); with
+ *     ckpt_every = 0 there is no recompute: tape is the full tape written by
+ *     cctamd_track_rk4_taped (double[max_steps][36][P]) and checkpoints may be NULL;
  *   adj: double[6][P], in: dJ/d(final x, v); out: dJ/d(initial x, v);
  *   tgrad: double[nblock][ntile][5] (+=): per tile sum B-bar.dB (current; divide by
  *     I) and dJ/dc_0..3 (tilt coefficients);
  *   qgrad: double[2*Q][P] (+=): dJ/d(gradient), dJ/d(second gradient) of each QS;
  *   workspaces: state_ws[6][P], scratch[12][P], partial (as cctamd_bs_field),
- *     tape[ckpt_every][36][P], adj_ws[15][P], bs_pts[3][P], bs_wts[3][P].
+ *     tape[ckpt_every][36][P] (or the full tape), adj_ws[15][P], bs_pts[3][P], bs_wts[3][P].
  */
 int cctamd_track_rk4_adjoint(int precision, int ppl, const void *seg, const double *tile_org,
                              const float *seg32, const float *seg_tan, const int32_t *tile_cct,

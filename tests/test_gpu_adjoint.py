@@ -188,3 +188,33 @@ def test_adjoint_split_wave_blocks_match_full_blocks():
         assert np.max(np.abs(final[:, :3] - res[0][0][:, :3])) < 1e-9
         assert np.max(np.abs(a0 - res[0][1])) <= 1e-6 * np.max(np.abs(res[0][1]))
         assert np.max(np.abs(cg - res[0][2])) <= 1e-5 * np.max(np.abs(res[0][2]))
+
+
+def test_adjoint_full_tape_matches_checkpoints():
+    """ckpt_every = 0 (the forward keeps every step's tape, no recompute) gives the bits of the
+    checkpointed reverse pass, for 64-point and split-wave blocks."""
+    from cctpy_amd import engine
+    wl = W.population(n_designs=2, n_particles=16, seed=6)
+    bls = [W.build_beamline(d) for d in wl.designs]
+    ps, npt, lengths = [], [], []
+    for i, (bl, beam) in enumerate(zip(bls, wl.beams)):
+        q = W.beam_particles(beam, bl)
+        ps += q
+        npt.append(len(q))
+        lengths += [0.5 + 0.1 * i] * len(q)
+    state = np.array([p.position.to_list() + p.velocity.to_list() for p in ps])
+    kq = np.array([p.e / p.relativistic_mass for p in ps])
+    sd = [engine.rk4_steps(L, 2e-3, p.speed) for L, p in zip(lengths, ps)]
+    ns = np.array([s for s, _ in sd], np.int64)
+    dt = np.array([d for _, d in sd])
+    w = np.random.default_rng(3).normal(size=(len(ps), 6)) * np.array([1.0, 1.0, 1.0, 1e-8, 1e-8, 1e-8])
+    for ppl in (1, -2):
+        out = []
+        for k in (16, 0):
+            tr = adjoint.AdjointTracker(bls, precision=33, ckpt_every=k, ppl=ppl)
+            final = tr.forward(state, kq, dt, ns, npt)
+            a0, cg, qg = tr.backward(w)
+            out.append((final, a0, cg))
+        assert np.array_equal(out[0][0], out[1][0])
+        assert np.array_equal(out[0][1], out[1][1])
+        assert np.array_equal(out[0][2], out[1][2])
