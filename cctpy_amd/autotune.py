@@ -47,9 +47,14 @@ def _store(key: str, val: dict) -> None:
         pass
 
 
+# Bumped whenever the kernels or the candidate set change, so that cached choices of an
+# older library are not reused.
+KERNEL_SET = 3
+
+
 def _key(model: "engine.DeviceModel", npt: Sequence[int], precision: int) -> str:
     props = torch.cuda.get_device_properties(engine.device())
-    return f"{props.name}|cu{props.multi_processor_count}|fp{precision}|" \
+    return f"v{KERNEL_SET}|{props.name}|cu{props.multi_processor_count}|fp{precision}|" \
            f"seg{list(map(int, model.design_nseg))}|pt{list(map(int, npt))}"
 
 
