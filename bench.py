@@ -50,9 +50,10 @@ def log(*a):
 
 
 def kernel_name(ppl: int) -> str:
-    """Name of the FP32 tracking kernel a points-per-lane code launches (32/16-point blocks:
-    4/8 lane groups with 2 points per lane)."""
-    return f"k_bs_partial_f32k<{ppl}>" if ppl > 0 else f"k_bs_partial_f32k_grp<{2 * -ppl}, 2>"
+    """Name of the FP32 tracking kernel a points-per-lane code launches (32-point blocks: rows of
+    16 lanes fed by DPP row broadcasts; 16-point blocks: 8 lane groups fed from LDS)."""
+    return f"k_bs_partial_f32k<{ppl}>" if ppl > 0 else ("k_bs_partial_f32k_row" if ppl == -2
+                                                         else "k_bs_partial_f32k_grp<8, 2>")
 
 
 def measured_traffic(workload: str, kernel: str, nwork: int):

@@ -296,6 +296,162 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k(const float* __restrict
     write_partial<PPL>(ax, ay, az, wave, lane, partial + partial_base(w, PB), w.nchunk);
 }
 
+// 32-point blocks without LDS: each 16-lane row of a wave is a lane group with
+// two points per lane that walks its own tiles.  Lane i of a row holds the K/C
+// rows of segments i and i + 16 of the row's tile in registers (coalesced
+// vector loads, the next tile prefetched), and segment k reaches the whole row
+// through the row-broadcast DPP move (lane k of each row to all 16 lanes): 10
+// moves per segment shared by 2 points, no LDS traffic and no barriers.
+template <int S>
+__device__ __forceinline__ float row_bcast(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + S, 0xf, 0xf,
+                                                                  true));
+}
+
+struct KCRow {
+    float4 a, b, c;  // kl.xyz, C.x | C.yz, -2m.xy | -2m.z, |m|^2, 0, 0
+};
+
+template <int S>
+__device__ __forceinline__ void kc_row_steps(const KCRow& g, const float (&qx)[2], const float (&qy)[2],
+                                             const float (&qz)[2], const float (&qq)[2], float (&kx)[2],
+                                             float (&ky)[2], float (&kz)[2], float (&cx)[2], float (&cy)[2],
+                                             float (&cz)[2]) {
+    const float klx = row_bcast<S>(g.a.x), kly = row_bcast<S>(g.a.y), klz = row_bcast<S>(g.a.z);
+    const float cmx = row_bcast<S>(g.a.w), cmy = row_bcast<S>(g.b.x), cmz = row_bcast<S>(g.b.y);
+    const float mx2 = row_bcast<S>(g.b.z), my2 = row_bcast<S>(g.b.w), mz2 = row_bcast<S>(g.c.x);
+    const float mm = row_bcast<S>(g.c.y);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const float r2 = __builtin_fmaf(qz[j], mz2, __builtin_fmaf(qy[j], my2, __builtin_fmaf(qx[j], mx2, qq[j] + mm)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float w3 = inv * inv * inv;
+        kx[j] += klx * w3;
+        ky[j] += kly * w3;
+        kz[j] += klz * w3;
+        cx[j] += cmx * w3;
+        cy[j] += cmy * w3;
+        cz[j] += cmz * w3;
+    }
+    if constexpr (S + 1 < 16) kc_row_steps<S + 1>(g, qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
+}
+
+template <int S>
+__device__ __forceinline__ void direct_row_steps(const KCRow& g, const float (&qx)[2], const float (&qy)[2],
+                                                 const float (&qz)[2], float (&bx)[2], float (&by)[2],
+                                                 float (&bz)[2]) {
+    const KCRow h{make_float4(row_bcast<S>(g.a.x), row_bcast<S>(g.a.y), row_bcast<S>(g.a.z), 0.0f),
+                  make_float4(0.0f, 0.0f, row_bcast<S>(g.b.z), row_bcast<S>(g.b.w)),
+                  make_float4(row_bcast<S>(g.c.x), 0.0f, 0.0f, 0.0f)};
+#pragma unroll
+    for (int j = 0; j < 2; ++j) tile_direct_term(h.a, h.b, h.c, qx[j], qy[j], qz[j], bx[j], by[j], bz[j]);
+    if constexpr (S + 1 < 16) direct_row_steps<S + 1>(g, qx, qy, qz, bx, by, bz);
+}
+
+__global__ __launch_bounds__(kWG) void k_bs_partial_f32k_row(const float* __restrict__ seg32k,
+                                                             const double* __restrict__ tile_org,
+                                                             const Work* __restrict__ work,
+                                                             const double* __restrict__ pts, int64_t ptot,
+                                                             double* __restrict__ partial) {
+    constexpr int LG = 16;                  // lanes per row
+    constexpr int PB = 2 * LG;              // 32 points per block
+    constexpr int NS = kWavesPerWG * 4;     // tile streams (rows) per workgroup
+    __shared__ double red[kWavesPerWG][3][PB];
+    const Work w = work[blockIdx.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int gl = lane & (LG - 1);
+    const int stream = wave * 4 + (lane >> 4);
+    double px[2], py[2], pz[2], ax[2], ay[2], az[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        int idx = gl + LG * j;
+        idx = idx < w.pt_count ? idx : 0;  // surplus lanes duplicate point 0, results discarded
+        const int64_t gp = (int64_t)w.pt_start + idx;
+        px[j] = pts[gp];
+        py[j] = pts[ptot + gp];
+        pz[j] = pts[2 * ptot + gp];
+        ax[j] = ay[j] = az[j] = 0.0;
+    }
+    const int tend = w.tile_start + w.tile_count;
+    const float4* src = reinterpret_cast<const float4*>(seg32k);
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    auto load = [&](int t, KCRow& g0, KCRow& g1) {  // segments gl and gl + 16 of tile t
+        const float4* r0 = src + ((int64_t)t * kTile + gl) * 3;
+        const float4* r1 = r0 + LG * 3;
+        g0 = KCRow{r0[0], r0[1], r0[2]};
+        g1 = KCRow{r1[0], r1[1], r1[2]};
+    };
+    int t = w.tile_start + stream;
+    KCRow g0{z4, z4, z4}, g1{z4, z4, z4};
+    if (t < tend) load(t, g0, g1);
+    while (t < tend) {  // uniform within a row
+        const int tn = t + NS;
+        KCRow n0{z4, z4, z4}, n1{z4, z4, z4};
+        if (tn < tend) load(tn, n0, n1);  // prefetch the row's next tile
+        const double ox = tile_org[4 * t], oy = tile_org[4 * t + 1], oz = tile_org[4 * t + 2];
+        const double near2 = kNearTile * tile_org[4 * t + 3];
+        float qx[2], qy[2], qz[2], qq[2], kx[2], ky[2], kz[2], cx[2], cy[2], cz[2];
+        bool near = false;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const double dx = px[j] - ox, dy = py[j] - oy, dz = pz[j] - oz;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            near = near || d2 < near2;
+            qx[j] = (float)dx;
+            qy[j] = (float)dy;
+            qz[j] = (float)dz;
+            qq[j] = (float)d2;
+            kx[j] = ky[j] = kz[j] = cx[j] = cy[j] = cz[j] = 0.0f;
+        }
+        if (__any(near)) {  // direct form for the active rows on this round
+            direct_row_steps<0>(g0, qx, qy, qz, kx, ky, kz);
+            direct_row_steps<0>(g1, qx, qy, qz, kx, ky, kz);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                ax[j] += (double)kx[j];
+                ay[j] += (double)ky[j];
+                az[j] += (double)kz[j];
+            }
+        } else {
+            kc_row_steps<0>(g0, qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
+            kc_row_steps<0>(g1, qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                ax[j] += (double)((ky[j] * qz[j] - kz[j] * qy[j]) - cx[j]);
+                ay[j] += (double)((kz[j] * qx[j] - kx[j] * qz[j]) - cy[j]);
+                az[j] += (double)((kx[j] * qy[j] - ky[j] * qx[j]) - cz[j]);
+            }
+        }
+        g0 = n0;
+        g1 = n1;
+        t = tn;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+        for (int off = LG; off < kWave; off <<= 1) {  // add the rows (fixed order)
+            ax[j] += __shfl_xor(ax[j], off);
+            ay[j] += __shfl_xor(ay[j], off);
+            az[j] += __shfl_xor(az[j], off);
+        }
+    }
+    if (lane < LG) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            red[wave][0][lane + LG * j] = ax[j];
+            red[wave][1][lane + LG * j] = ay[j];
+            red[wave][2][lane + LG * j] = az[j];
+        }
+    }
+    __syncthreads();
+    double* out = partial + partial_base(w, PB);
+    for (int k = threadIdx.x; k < 3 * PB; k += kWG) {
+        const int c = k / PB, i = k % PB;
+        out[(int64_t)(i * 3 + c) * w.nchunk] = ((red[0][c][i] + red[1][c][i]) + red[2][c][i]) + red[3][c][i];
+    }
+}
+
 // Small point blocks (32 or 16 points): each wave is split into G lane groups
 // that hold the same points (PL per lane) and walk different tiles, so every
 // lane works.  Each wave reads 3 ds_read_b128 per segment whatever PL is, and
@@ -630,7 +786,7 @@ hipError_t launch_bs_any(int precision, int ppl, const void* seg, const double* 
         if (nwork == 0) return hipSuccess;
         if (precision != 33) return hipErrorInvalidValue;
         if (ppl == -2)
-            hipLaunchKernelGGL((k_bs_partial_f32k_grp<4, 2>), dim3((unsigned)nwork), dim3(kWG), 0, st,
+            hipLaunchKernelGGL(k_bs_partial_f32k_row, dim3((unsigned)nwork), dim3(kWG), 0, st,
                                (const float*)seg, tile_org, work, pts, ptot, partial);
         else
             hipLaunchKernelGGL((k_bs_partial_f32k_grp<8, 2>), dim3((unsigned)nwork), dim3(kWG), 0, st,
