@@ -648,15 +648,20 @@ __global__ __launch_bounds__(kWG) void k_field_reduce(int ppl, const double* __r
 }
 
 // scratch rows: 0-2 stage position (read by k_bs_partial), 3-5 stage
-// velocity, 6-8 running sum of position slopes, 9-11 of velocity slopes.
+// velocity, 6-8 running sum of position slopes, 9-11 of velocity slopes,
+// 12-13 the particle's partial-sum slot (so that stage kernels load it at once).
 __global__ void k_rk4_init(const double* __restrict__ state, int64_t ptot, double* __restrict__ scratch,
-                           double* __restrict__ record) {
+                           double* __restrict__ record, int ppl, const double* __restrict__ partial,
+                           const int64_t* __restrict__ design_wbase, const int32_t* __restrict__ design_nchunk,
+                           const int64_t* __restrict__ design_pt_off, int64_t nd) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ptot) return;
     for (int c = 0; c < 6; ++c) {
         scratch[c * ptot + g] = state[c * ptot + g];
         if (record) record[c * ptot + g] = state[c * ptot + g];
     }
+    store_slot(scratch, ptot, g, point_slot(g, ppl, partial, design_wbase, design_nchunk, design_pt_off, nd),
+               partial);
 }
 
 #pragma clang fp contract(off)
@@ -698,7 +703,7 @@ __global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __rest
             sv[c] = kv[o];
         }
     }
-    const PointSlot sl = point_slot(g, ppl, partial, design_wbase, design_nchunk, design_pt_off, nd);
+    const PointSlot sl = load_slot(scratch, ptot, g, partial);
     if (!active) return;  // finished particles keep their state
     double b[3];
     chunk_sum(sl, b);
@@ -806,7 +811,8 @@ hipError_t launch_rk4_steps(const DeviceSet& ds, double* state, int64_t ptot, co
     const int64_t eblocks = ceil_div(ptot, 256);
     const int64_t rblocks = ceil_div(ptot, kWavesPerWG);
     hipLaunchKernelGGL(k_rk4_init, dim3((unsigned)eblocks), dim3(256), 0, st, state, ptot, scratch,
-                       step0 == 0 ? record : nullptr);
+                       step0 == 0 ? record : nullptr, ds.ppl, partial, ds.design_wbase, ds.design_nchunk,
+                       ds.design_pt_off, ds.ndesign);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     for (int64_t step = step0; step < step1; ++step) {

@@ -152,6 +152,25 @@ __device__ inline PointSlot point_slot(int64_t g, int ppl, const double* __restr
     return s;
 }
 
+// The same slot precomputed per particle: scratch rows 12 (offset of the
+// particle's partial rows, int64 bits) and 13 (nchunk | design << 32).
+__device__ inline void store_slot(double* __restrict__ scratch, int64_t ptot, int64_t g, const PointSlot& s,
+                                  const double* __restrict__ partial) {
+    scratch[12 * ptot + g] = __longlong_as_double((long long)(s.rows - partial));
+    scratch[13 * ptot + g] = __longlong_as_double(((long long)s.design << 32) | (long long)(unsigned)s.nc);
+}
+
+__device__ inline PointSlot load_slot(const double* __restrict__ scratch, int64_t ptot, int64_t g,
+                                      const double* __restrict__ partial) {
+    PointSlot s;
+    const long long off = __double_as_longlong(scratch[12 * ptot + g]);
+    const long long dn = __double_as_longlong(scratch[13 * ptot + g]);
+    s.rows = partial + off;
+    s.nc = (int)(unsigned)(dn & 0xffffffffll);
+    s.design = dn >> 32;
+    return s;
+}
+
 // Fixed-order sum over chunks (lanes stride the chunks, then a butterfly):
 // deterministic, and every lane ends with the total.
 __device__ inline void chunk_sum(const PointSlot& s, double b[3]) {

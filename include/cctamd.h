@@ -131,7 +131,7 @@ int cctamd_bs_partial(int precision, int ppl, const void *seg, const double *til
  * (final_code/packages/gpu_accelerator.py:1912-2044) and the CPU ParticleRunner.run_only
  * (final_code/packages/particles.py:561-632 with base_utils.py:263-311).
  *   state    double[6][P] in/out: x, y, z, vx, vy, vz rows.
- *   scratch  double[12][P] device workspace.
+ *   scratch  double[14][P] device workspace.
  *   kq[P] = e/m, dt[P], nsteps[P] (int64) per particle; steps beyond nsteps[i]
  *   leave particle i unchanged; max_steps = max(nsteps).
  *   record   nullable double[max_steps/record_every+1][6][P]: the state at step 0 and
@@ -187,7 +187,7 @@ int cctamd_track_rk4_taped(int precision, int ppl, const void *seg, const double
  *   tgrad: double[nblock][ntile][5] (+=): per tile sum B-bar.dB (current; divide by
  *     I) and dJ/dc_0..3 (tilt coefficients);
  *   qgrad: double[2*Q][P] (+=): dJ/d(gradient), dJ/d(second gradient) of each QS;
- *   workspaces: state_ws[6][P], scratch[12][P], partial (as cctamd_bs_field),
+ *   workspaces: state_ws[6][P], scratch[14][P], partial (as cctamd_bs_field),
  *     tape[ckpt_every][36][P] (or the full tape), adj_ws[15][P], bs_pts[3][P], bs_wts[3][P].
  */
 int cctamd_track_rk4_adjoint(int precision, int ppl, const void *seg, const double *tile_org,
