@@ -325,7 +325,8 @@ def main() -> int:
         rate64 = S.particle_steps / (ms64 / 1e3)
         d = np.linalg.norm(out32[:, :3] - out64[:, :3], axis=1)
         err = {"max": float(d.max() * 1e6), "rms": float(np.sqrt(np.mean(d ** 2)) * 1e6),
-               "p99": float(np.percentile(d, 99) * 1e6)}
+               "p99": float(np.percentile(d, 99) * 1e6), "median": float(np.median(d) * 1e6),
+               "within_1um": float(np.mean(d <= 1e-6))}
         fp64 = {"particle_steps_per_s": rate64, "pair_evals_per_s": S.pairs_per_pass / (ms64 / 1e3),
                 "ms_per_step": ms64, "ppl": tuned[64]["ppl"]}
         log(f"[bench] fp64: {ms64:.1f} ms/pass, {rate64:.4g} particle-steps/s; fp32 position error "
