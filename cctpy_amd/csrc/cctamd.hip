@@ -839,7 +839,7 @@ hipError_t launch_rk4_steps(const DeviceSet& ds, double* state, int64_t ptot, co
 using namespace cctamd_detail;
 
 extern "C" {
-int cctamd_version(void) { return 100; }
+int cctamd_version(void) { return 200; }
 
 const char* cctamd_error_string(int code) {
     if (code == CCTAMD_OK) return "ok";

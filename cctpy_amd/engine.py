@@ -6,7 +6,8 @@ a ``DeviceModel`` holds D designs at once:
 * CCT windings are generated on the GPU (``cctamd_winding_generate``) straight
   into an FP64 AoS segment array in the global frame, each design padded to a
   multiple of 32 segments (one *tile*), and packed once into FP32 tiles
-  re-centred on a per-tile FP64 origin (``cctamd_pack_fp32``);
+  re-centred on an FP64 tile origin, the mean midpoint of the tile
+  (``cctamd_pack_fp32``);
 * QS magnets travel as the reference's 16-double records, uniform magnets as a
   per-design constant field.
 

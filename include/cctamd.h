@@ -12,7 +12,9 @@
  *              kl = 1e-7*I*(P[i+1]-P[i]), m = (P[i+1]+P[i])/2; S is a multiple of
  *              CCTAMD_TILE; each design's range is padded with kl=0 segments.
  *   seg32      float[S][8]   {klx, kly, klz, 0, m-c, 0} with c = tile_org[tile]
- *   tile_org   double[S/CCTAMD_TILE][4]  per-tile FP64 origin (first midpoint)
+ *   seg32k     float[S][12]  {kl, kl x (m-c), -2(m-c), |m-c|^2, 0, 0}
+ *   tile_org   double[S/CCTAMD_TILE][4]  {c, e2}: FP64 tile origin (mean of the
+ *              tile's current-carrying midpoints) and extent max |m-c|^2
  *   qs         double[Q][16] {O, XI, YI, ZI, length, gradient, second_gradient,
  *              aperture_radius} (the reference's QS record)
  *   bconst     double[D][4]  uniform field per design (sum of uniform magnets)
