@@ -21,12 +21,13 @@
 namespace cctamd_detail {
 namespace {
 
-constexpr int kGradTilts = 4;   // tilt slots carried by the adjoint
-constexpr int kTanFloats = 16;  // per segment: T_i.xyz, s_i[4], 0, T_i+1.xyz, s_i+1[4], 0
+constexpr int kGradTilts = 4;          // tilt slots carried by the adjoint
+constexpr int kTanRows = kTile + 1;    // winding points per tile in the tangent table
 constexpr int kTileGrad = 1 + kGradTilts;
 
 // ---------------------------------------------------------- tangents
-// dP/dphi at ksi in the local frame: (-t sinh(eta) sin(phi), t sinh(eta) cos(phi), 0).
+// dP/dphi at ksi in the global frame ((-t sinh(eta) sin(phi), t sinh(eta) cos(phi), 0)
+// locally) and sin((j+1) ksi), j = 0..3.
 __device__ inline void cct_tangent(const double* __restrict__ prm, int64_t i, float out[8]) {
     const double ksi = prm[15] + prm[16] * (double)i;
     const double phi = cct_phi(prm, ksi);
@@ -40,21 +41,36 @@ __device__ inline void cct_tangent(const double* __restrict__ prm, int64_t i, fl
     out[7] = 0.0f;
 }
 
+__device__ inline void put_tangent(float* __restrict__ seg_tan, int64_t tile, int entry, const float v[8],
+                                   float real) {
+    float* o = seg_tan + (tile * kTanRows + entry) * 8;
+    for (int c = 0; c < 7; ++c) o[c] = v[c];
+    o[7] = real;
+}
+
+// One thread per winding point (count + 1 per CCT; every CCT starts on a tile
+// boundary). Point i of a CCT whose segments start at row dst is the start of
+// segment i (entry (dst+i) % kTile of its tile, real = 1) and, when segment i-1
+// ends a tile, that tile's entry kTile; the end point of the last segment also
+// fills the first pad slot of a partial tile (real = 0).
 __global__ void k_winding_tangents(const double* __restrict__ params, const int64_t* __restrict__ count_off,
-                                   const int64_t* __restrict__ dst, int64_t ncct, int64_t nseg_total,
+                                   const int64_t* __restrict__ dst, int64_t ncct, int64_t npoint_total,
                                    float* __restrict__ seg_tan) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nseg_total) return;
-    int64_t lo = 0, hi = ncct - 1;
+    if (g >= npoint_total) return;
+    int64_t lo = 0, hi = ncct - 1;  // last c with count_off[c] + c <= g
     while (lo < hi) {
         const int64_t mid = (lo + hi + 1) >> 1;
-        if (count_off[mid] <= g) lo = mid; else hi = mid - 1;
+        if (count_off[mid] + mid <= g) lo = mid; else hi = mid - 1;
     }
     const double* prm = params + lo * CCTAMD_CCT_PARAMS;
-    const int64_t i = g - count_off[lo];
-    float* out = seg_tan + (dst[lo] + i) * kTanFloats;
-    cct_tangent(prm, i, out);
-    cct_tangent(prm, i + 1, out + 8);
+    const int64_t i = g - count_off[lo] - lo;
+    const int64_t n = count_off[lo + 1] - count_off[lo];
+    float v[8];
+    cct_tangent(prm, i, v);
+    const int64_t u = dst[lo] + i;
+    if (i < n || u % kTile != 0) put_tangent(seg_tan, u / kTile, (int)(u % kTile), v, i < n ? 1.0f : 0.0f);
+    if (i >= 1 && u % kTile == 0) put_tangent(seg_tan, u / kTile - 1, kTile, v, 0.0f);
 }
 
 __device__ inline float wave_sum(float v) {
@@ -64,13 +80,79 @@ __device__ inline float wave_sum(float v) {
 }
 
 // -------------------------------------------------- backward Biot-Savart
-// For each point p with weight Bb (the adjoint of the field at p):
-//   pbar(p)  = sum_i (Bb x kl_i) w_i - 3 r_i (Bb . (kl_i x r_i)) w_i / |r_i|^2,  w = |r|^-3
-// per segment i (summed over the points of the wave):
-//   dJ/dkl_i = sum_p (r x Bb) w,   dJ/dm_i = -sum_p u_p,i   (u = the pbar term)
-// contracted at once with the winding tangents into per-tile accumulators
-// tgrad[block][tile] = {sum Bb . dB (current), dJ/dc_0..3 (tilt coefficients)}.
+// For each point p with weight b (the adjoint of the field at p) and segment i,
+// r = p - m_i, w = |r|^-3:
+//   point side:   pbar(p) += u = w (b x kl_i) - 3 w |r|^-2 (kl_i . (r x b)) r
+//   segment side: dJ/dkl_i += w (r x b),  dJ/dm_i -= u,  dJ/dI += w kl_i . (r x b) / I
+// The segment sums are contracted at once with the winding tangents: with
+// kl_i = coef (P_i+1 - P_i), m_i = (P_i + P_i+1)/2 and dP_i/dc_j = T_i s_ij,
+//   dJ/dc_j = sum over winding points i of s_ij T_i . (A_i-1 - B_i),
+//   A = coef dJ/dkl + dJ/dm / 2,  B = coef dJ/dkl - dJ/dm / 2,
+// so each point's end-of-segment and start-of-segment shares take one dot product.
+// tgrad[block][tile] = {sum b . dB (current), dJ/dc_0..3 (tilt coefficients)}.
 // FP32 terms on re-centred tiles, FP64 across tiles.
+
+// One point x segment term. hb = b / 2; returns c = r x b, w = |r|^-3,
+// h = u / 2 and e = w kl . c.
+struct BwdTerm {
+    float cx, cy, cz, w, hx, hy, hz, e;
+};
+
+__device__ __forceinline__ BwdTerm bwd_term(float qx, float qy, float qz, float bx, float by, float bz, float hbx,
+                                            float hby, float hbz, const float4 kl, const float4 m) {
+    BwdTerm o;
+    const float rx = qx - m.x, ry = qy - m.y, rz = qz - m.z;
+    const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float inv2 = inv * inv;
+    o.w = inv2 * inv;
+    o.cx = __builtin_fmaf(ry, bz, -(rz * by));
+    o.cy = __builtin_fmaf(rz, bx, -(rx * bz));
+    o.cz = __builtin_fmaf(rx, by, -(ry * bx));
+    const float sdot = __builtin_fmaf(kl.z, o.cz, __builtin_fmaf(kl.y, o.cy, kl.x * o.cx));
+    o.e = sdot * o.w;
+    const float na = -1.5f * (o.e * inv2);
+    const float kx = __builtin_fmaf(hby, kl.z, -(hbz * kl.y));
+    const float ky = __builtin_fmaf(hbz, kl.x, -(hbx * kl.z));
+    const float kz = __builtin_fmaf(hbx, kl.y, -(hby * kl.x));
+    o.hx = __builtin_fmaf(na, rx, o.w * kx);
+    o.hy = __builtin_fmaf(na, ry, o.w * ky);
+    o.hz = __builtin_fmaf(na, rz, o.w * kz);
+    return o;
+}
+
+// Segment-side step of the tangent contraction for one segment: gk = dJ/dkl and
+// gh = -dJ/dm / 2 summed over this lane's points, ck = coef of the segment (0 for a
+// pad slot), ta/tb = tangent entry of its start point. Adds the start point's share
+// (A_prev - B) . T s_j to g and leaves A of this segment in a.
+struct TanAcc {
+    float ax, ay, az, g0, g1, g2, g3;
+};
+
+__device__ __forceinline__ void tan_step(TanAcc& t, float ck, float gkx, float gky, float gkz, float ghx, float ghy,
+                                         float ghz, const float4 ta, const float4 tb) {
+    const float dx = t.ax - __builtin_fmaf(ck, gkx, ghx);
+    const float dy = t.ay - __builtin_fmaf(ck, gky, ghy);
+    const float dz = t.az - __builtin_fmaf(ck, gkz, ghz);
+    const float d = __builtin_fmaf(dz, ta.z, __builtin_fmaf(dy, ta.y, dx * ta.x));
+    t.g0 = __builtin_fmaf(ta.w, d, t.g0);
+    t.g1 = __builtin_fmaf(tb.x, d, t.g1);
+    t.g2 = __builtin_fmaf(tb.y, d, t.g2);
+    t.g3 = __builtin_fmaf(tb.z, d, t.g3);
+    t.ax = __builtin_fmaf(ck, gkx, -ghx);
+    t.ay = __builtin_fmaf(ck, gky, -ghy);
+    t.az = __builtin_fmaf(ck, gkz, -ghz);
+}
+
+// End point of the tile's last segment (entry kTile).
+__device__ __forceinline__ void tan_end(TanAcc& t, const float4 ta, const float4 tb) {
+    const float d = __builtin_fmaf(t.az, ta.z, __builtin_fmaf(t.ay, ta.y, t.ax * ta.x));
+    t.g0 = __builtin_fmaf(ta.w, d, t.g0);
+    t.g1 = __builtin_fmaf(tb.x, d, t.g1);
+    t.g2 = __builtin_fmaf(tb.y, d, t.g2);
+    t.g3 = __builtin_fmaf(tb.z, d, t.g3);
+}
+
 template <int PPL>
 __global__ __launch_bounds__(kWG) void k_bs_backward_f32(
     const float* __restrict__ seg32, const double* __restrict__ tile_org, const float* __restrict__ seg_tan,
@@ -88,12 +170,15 @@ __global__ __launch_bounds__(kWG) void k_bs_backward_f32(
     for (int j = 0; j < PPL; ++j) {  // surplus lanes carry no weight
         if (lane + kWave * j >= w.pt_count) wx[j] = wy[j] = wz[j] = 0.0;
     }
-    float bx[PPL], by[PPL], bz[PPL];
+    float bx[PPL], by[PPL], bz[PPL], hbx[PPL], hby[PPL], hbz[PPL];
 #pragma unroll
     for (int j = 0; j < PPL; ++j) {
         bx[j] = (float)wx[j];
         by[j] = (float)wy[j];
         bz[j] = (float)wz[j];
+        hbx[j] = 0.5f * bx[j];
+        hby[j] = 0.5f * by[j];
+        hbz[j] = 0.5f * bz[j];
     }
     double ax[PPL], ay[PPL], az[PPL];
 #pragma unroll
@@ -112,59 +197,50 @@ __global__ __launch_bounds__(kWG) void k_bs_backward_f32(
             qz[j] = (float)(pz[j] - oz);
             sx[j] = sy[j] = sz[j] = 0.0f;
         }
-        float cur = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+        float cur = 0.0f;
+        TanAcc ta{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         const float4* s = reinterpret_cast<const float4*>(seg32) + (int64_t)t * kTile * 2;
-        const float4* tn = reinterpret_cast<const float4*>(seg_tan) + (int64_t)t * kTile * 4;
+        const float4* tn = reinterpret_cast<const float4*>(seg_tan) + (int64_t)t * kTanRows * 2;
 #pragma unroll 2
         for (int k = 0; k < kTile; ++k) {
             const float4 kl = s[2 * k];
             const float4 m = s[2 * k + 1];
-            float gkx = 0.0f, gky = 0.0f, gkz = 0.0f, gmx = 0.0f, gmy = 0.0f, gmz = 0.0f;
+            float gkx, gky, gkz, ghx, ghy, ghz;
 #pragma unroll
             for (int j = 0; j < PPL; ++j) {
-                const float rx = qx[j] - m.x, ry = qy[j] - m.y, rz = qz[j] - m.z;
-                const float r2 = rx * rx + ry * ry + rz * rz;
-                const float inv = __builtin_amdgcn_rsqf(r2);
-                const float w3 = inv * inv * inv;
-                const float tx = kl.y * rz - kl.z * ry, ty = kl.z * rx - kl.x * rz, tz = kl.x * ry - kl.y * rx;
-                const float sdot = bx[j] * tx + by[j] * ty + bz[j] * tz;
-                const float a3 = 3.0f * sdot * w3 * (inv * inv);
-                const float ux = (by[j] * kl.z - bz[j] * kl.y) * w3 - rx * a3;
-                const float uy = (bz[j] * kl.x - bx[j] * kl.z) * w3 - ry * a3;
-                const float uz = (bx[j] * kl.y - by[j] * kl.x) * w3 - rz * a3;
-                sx[j] += ux;
-                sy[j] += uy;
-                sz[j] += uz;
-                gmx -= ux;
-                gmy -= uy;
-                gmz -= uz;
-                gkx += (ry * bz[j] - rz * by[j]) * w3;
-                gky += (rz * bx[j] - rx * bz[j]) * w3;
-                gkz += (rx * by[j] - ry * bx[j]) * w3;
-                cur += sdot * w3;
+                const BwdTerm b = bwd_term(qx[j], qy[j], qz[j], bx[j], by[j], bz[j], hbx[j], hby[j], hbz[j], kl, m);
+                sx[j] += b.hx;
+                sy[j] += b.hy;
+                sz[j] += b.hz;
+                cur += b.e;
+                if (j == 0) {
+                    gkx = b.w * b.cx;
+                    gky = b.w * b.cy;
+                    gkz = b.w * b.cz;
+                    ghx = b.hx;
+                    ghy = b.hy;
+                    ghz = b.hz;
+                } else {
+                    gkx = __builtin_fmaf(b.w, b.cx, gkx);
+                    gky = __builtin_fmaf(b.w, b.cy, gky);
+                    gkz = __builtin_fmaf(b.w, b.cz, gkz);
+                    ghx += b.hx;
+                    ghy += b.hy;
+                    ghz += b.hz;
+                }
             }
-            // d kl/dc_j = coef (T' s'_j - T s_j), d m/dc_j = (T' s'_j + T s_j)/2
-            const float4 t0 = tn[4 * k], t1 = tn[4 * k + 1], t2 = tn[4 * k + 2], t3 = tn[4 * k + 3];
-            const float Ax = coef * gkx + 0.5f * gmx, Ay = coef * gky + 0.5f * gmy, Az = coef * gkz + 0.5f * gmz;
-            const float Bx = 0.5f * gmx - coef * gkx, By = 0.5f * gmy - coef * gky, Bz = 0.5f * gmz - coef * gkz;
-            const float alpha = Ax * t2.x + Ay * t2.y + Az * t2.z;  // at the segment's end point
-            const float beta = Bx * t0.x + By * t0.y + Bz * t0.z;   // at its start point
-            g0 += t2.w * alpha + t0.w * beta;
-            g1 += t3.x * alpha + t1.x * beta;
-            g2 += t3.y * alpha + t1.y * beta;
-            g3 += t3.z * alpha + t1.z * beta;
+            const float4 t0 = tn[2 * k], t1 = tn[2 * k + 1];
+            tan_step(ta, coef * t1.w, gkx, gky, gkz, ghx, ghy, ghz, t0, t1);
         }
+        tan_end(ta, tn[2 * kTile], tn[2 * kTile + 1]);
 #pragma unroll
-        for (int j = 0; j < PPL; ++j) {
-            ax[j] += (double)sx[j];
-            ay[j] += (double)sy[j];
-            az[j] += (double)sz[j];
+        for (int j = 0; j < PPL; ++j) {  // s holds u / 2
+            ax[j] += 2.0 * (double)sx[j];
+            ay[j] += 2.0 * (double)sy[j];
+            az[j] += 2.0 * (double)sz[j];
         }
         cur = wave_sum(cur);
-        g0 = wave_sum(g0);
-        g1 = wave_sum(g1);
-        g2 = wave_sum(g2);
-        g3 = wave_sum(g3);
+        const float g0 = wave_sum(ta.g0), g1 = wave_sum(ta.g1), g2 = wave_sum(ta.g2), g3 = wave_sum(ta.g3);
         if (lane == 0 && cct >= 0) {
             double* tg = tgrad + ((int64_t)w.block * ntile_total + t) * kTileGrad;
             tg[0] += (double)cur;
@@ -190,7 +266,7 @@ __global__ __launch_bounds__(kWG) void k_bs_backward_f32_grp(
     constexpr int PB = kWave / G;
     constexpr int NS = kWavesPerWG * G;  // tile streams per workgroup
     constexpr int SROW = kTile * 2;      // float4 per tile of seg32
-    constexpr int TROW = kTile * 4;      // float4 per tile of tangents
+    constexpr int TROW = kTanRows * 2;   // float4 per tile of tangents
     __shared__ float4 lseg[NS][SROW + 1];
     __shared__ float4 ltan[NS][TROW + 1];
     __shared__ double red[kWavesPerWG][3][PB];
@@ -205,6 +281,7 @@ __global__ __launch_bounds__(kWG) void k_bs_backward_f32_grp(
     const float bx = has ? (float)wts[gp] : 0.0f;  // surplus lanes carry no weight
     const float by = has ? (float)wts[ptot + gp] : 0.0f;
     const float bz = has ? (float)wts[2 * ptot + gp] : 0.0f;
+    const float hbx = 0.5f * bx, hby = 0.5f * by, hbz = 0.5f * bz;
     double ax = 0.0, ay = 0.0, az = 0.0;
     const int tend = w.tile_start + w.tile_count;
     const float4* s4 = reinterpret_cast<const float4*>(seg32);
@@ -227,44 +304,25 @@ __global__ __launch_bounds__(kWG) void k_bs_backward_f32_grp(
             const float qx = (float)(px - tile_org[4 * t]);
             const float qy = (float)(py - tile_org[4 * t + 1]);
             const float qz = (float)(pz - tile_org[4 * t + 2]);
-            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-            float cur = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f, cur = 0.0f;
+            TanAcc ta{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             const float4* sr = lseg[stream];
             const float4* tr = ltan[stream];
 #pragma unroll 2
             for (int k = 0; k < kTile; ++k) {
-                const float4 kl = sr[2 * k];
-                const float4 m = sr[2 * k + 1];
-                const float rx = qx - m.x, ry = qy - m.y, rz = qz - m.z;
-                const float r2 = rx * rx + ry * ry + rz * rz;
-                const float inv = __builtin_amdgcn_rsqf(r2);
-                const float w3 = inv * inv * inv;
-                const float tx = kl.y * rz - kl.z * ry, ty = kl.z * rx - kl.x * rz, tz = kl.x * ry - kl.y * rx;
-                const float sdot = bx * tx + by * ty + bz * tz;
-                const float a3 = 3.0f * sdot * w3 * (inv * inv);
-                const float ux = (by * kl.z - bz * kl.y) * w3 - rx * a3;
-                const float uy = (bz * kl.x - bx * kl.z) * w3 - ry * a3;
-                const float uz = (bx * kl.y - by * kl.x) * w3 - rz * a3;
-                sx += ux;
-                sy += uy;
-                sz += uz;
-                const float gkx = (ry * bz - rz * by) * w3, gky = (rz * bx - rx * bz) * w3,
-                            gkz = (rx * by - ry * bx) * w3;
-                cur += sdot * w3;
-                // d kl/dc_j = coef (T' s'_j - T s_j), d m/dc_j = (T' s'_j + T s_j)/2, dJ/dm = -u
-                const float4 ta = tr[4 * k], tb = tr[4 * k + 1], tc = tr[4 * k + 2], td = tr[4 * k + 3];
-                const float Ax = coef * gkx - 0.5f * ux, Ay = coef * gky - 0.5f * uy, Az = coef * gkz - 0.5f * uz;
-                const float Bx = -0.5f * ux - coef * gkx, By = -0.5f * uy - coef * gky, Bz = -0.5f * uz - coef * gkz;
-                const float alpha = Ax * tc.x + Ay * tc.y + Az * tc.z;
-                const float beta = Bx * ta.x + By * ta.y + Bz * ta.z;
-                g0 += tc.w * alpha + ta.w * beta;
-                g1 += td.x * alpha + tb.x * beta;
-                g2 += td.y * alpha + tb.y * beta;
-                g3 += td.z * alpha + tb.z * beta;
+                const BwdTerm b = bwd_term(qx, qy, qz, bx, by, bz, hbx, hby, hbz, sr[2 * k], sr[2 * k + 1]);
+                sx += b.hx;
+                sy += b.hy;
+                sz += b.hz;
+                cur += b.e;
+                const float4 e0 = tr[2 * k], e1 = tr[2 * k + 1];
+                tan_step(ta, coef * e1.w, b.w * b.cx, b.w * b.cy, b.w * b.cz, b.hx, b.hy, b.hz, e0, e1);
             }
-            ax += (double)sx;
-            ay += (double)sy;
-            az += (double)sz;
+            tan_end(ta, tr[2 * kTile], tr[2 * kTile + 1]);
+            ax += 2.0 * (double)sx;  // s holds u / 2
+            ay += 2.0 * (double)sy;
+            az += 2.0 * (double)sz;
+            float g0 = ta.g0, g1 = ta.g1, g2 = ta.g2, g3 = ta.g3;
 #pragma unroll
             for (int off = PB / 2; off > 0; off >>= 1) {  // sum over the group's points
                 cur += __shfl_xor(cur, off);
@@ -500,10 +558,11 @@ int cctamd_winding_tangents(const double* params, const int64_t* count_off, cons
     if (ncct < 0 || nseg_total < 0) return CCTAMD_EINVAL;
     if (ncct == 0 || nseg_total == 0) return CCTAMD_OK;
     if (!params || !count_off || !dst || !seg_tan) return CCTAMD_EINVAL;
-    const int64_t blocks = ceil_div(nseg_total, 256);
+    const int64_t npoint = nseg_total + ncct;  // count + 1 winding points per CCT
+    const int64_t blocks = ceil_div(npoint, 256);
     if (blocks > 0x7fffffff) return CCTAMD_ERANGE;
     hipLaunchKernelGGL(k_winding_tangents, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, params,
-                       count_off, dst, ncct, nseg_total, seg_tan);
+                       count_off, dst, ncct, npoint, seg_tan);
     return hip_status(hipGetLastError());
 }
 

@@ -177,19 +177,19 @@ class DeviceModel:
 
     # --------------------------------------------------------- adjoint data
     def adjoint_data(self):
-        """(seg_tan [S][16] f32, tile_cct [T] i32, cct_coef [ncct] f64) for cctamd_track_rk4_adjoint."""
+        """(seg_tan [T][33][8] f32, tile_cct [T] i32, cct_coef [ncct] f64) for cctamd_track_rk4_adjoint."""
         if not self.align_cct:
             raise ValueError("the adjoint needs a model built with align_cct=True")
         if getattr(self, "_adj", None) is None:
             dev = device()
             S = self.nseg_padded
-            tan = torch.zeros((max(S, 1), 16), dtype=torch.float32, device=dev)
+            nt = max(S // N.TILE, 1)
+            tan = torch.zeros((nt, N.TILE + 1, 8), dtype=torch.float32, device=dev)
             if self.ncct:
                 prm, off, dsts = self._gen
                 N.check(N.lib.cctamd_winding_tangents(_p(prm), _p(off), _p(dsts), self.ncct,
                                                       int(self.cct_counts.sum()), _p(tan), _stream()),
                         "cctamd_winding_tangents")
-            nt = max(S // N.TILE, 1)
             tc = numpy.full(nt, -1, numpy.int32)
             for c in range(self.ncct):
                 t0 = int(self.cct_dst[c]) // N.TILE

@@ -149,10 +149,15 @@ int cctamd_track_rk4(int precision, int ppl, const void *seg, const double *tile
                      double *record, int64_t record_every, void *stream);
 
 /*
- * Winding tangents for the adjoint: per segment 16 floats
- * {dP/dphi(ksi_i).xyz, sin((j+1) ksi_i) j=0..3, 0, dP/dphi(ksi_i+1).xyz, sin((j+1) ksi_i+1), 0}
- * in the global frame (dP/dc_j = dP/dphi * sin((j+1) ksi), c_j the tilt coefficient of
- * final_code/packages/cct.py:179-187).  Arguments as for cctamd_winding_generate.
+ * Winding tangents for the adjoint, per tile of a segment set whose CCTs each start
+ * on a tile boundary: float[ntile][33][8]; entry k < 32 is the winding point at the
+ * start of the tile's segment k, entry 32 the end point of segment 31:
+ * {dP/dphi(ksi).xyz, sin((j+1) ksi) j=0..3, real} in the global frame
+ * (dP/dc_j = dP/dphi * sin((j+1) ksi), c_j the tilt coefficient of
+ * final_code/packages/cct.py:179-187); real = 1 when segment k is a winding segment,
+ * 0 for entry 32 and pad slots (the end point of a CCT's last segment also fills
+ * the first pad slot of a partial tile).  The caller zero-fills seg_tan first.
+ * Arguments as for cctamd_winding_generate (nseg_total = sum of the counts).
  */
 int cctamd_winding_tangents(const double *params, const int64_t *count_off, const int64_t *dst,
                             int64_t ncct, int64_t nseg_total, float *seg_tan, void *stream);
