@@ -49,11 +49,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+BS_WAVES = 8                 # waves per workgroup of k_bs_partial_f32k (CCTAMD_BS_WAVES in csrc/cctamd.hip)
+
+
 def kernel_name(ppl: int) -> str:
     """Name of the FP32 tracking kernel a points-per-lane code launches (32-point blocks: rows of
     16 lanes fed by DPP row broadcasts; 16-point blocks: 8 lane groups fed from LDS)."""
-    return f"k_bs_partial_f32k<{ppl}>" if ppl > 0 else ("k_bs_partial_f32k_row" if ppl == -2
-                                                         else "k_bs_partial_f32k_grp<8, 2>")
+    return f"k_bs_partial_f32k<{ppl}, {BS_WAVES}>" if ppl > 0 else (
+        "k_bs_partial_f32k_row" if ppl == -2 else "k_bs_partial_f32k_grp<8, 2>")
 
 
 def measured_traffic(workload: str, kernel: str, nwork: int):
@@ -247,6 +250,43 @@ class Setup:
                 "grad_finite": bool(torch.isfinite(tr.tgrad).all().item())}
 
 
+def gradient_check(n_particles: int = 16) -> dict:
+    """dJ/dtheta by the adjoint (FP32 Biot-Savart, full tape) for a linear functional of the final
+    states, over the 10-parameter design vector of the adjoint_section workload (full geometry and
+    track, fewer protons), against central differences (h = 1e-6 relative) of the FP64 forward."""
+    import numpy as np
+    from cctpy_amd import adjoint
+    from cctpy_amd import workloads as W
+    wl = W.adjoint_section(n_particles=n_particles)
+    beam, length = wl.beams[0], wl.beam_length(0)
+
+    def build(th):
+        return W.build_beamline(W.section_ops(th))
+
+    def particles(bl):
+        return W.beam_particles(beam, bl)
+
+    theta = W.section_theta0()
+    t0 = time.perf_counter()
+    prob = adjoint.DesignProblem(build, particles, length, wl.footstep, precision=FP32, ckpt_every=0)
+    final = prob.forward(theta)
+    w = np.random.default_rng(11).normal(size=final.shape) * np.array([1.0, 1.0, 1.0, 1e-8, 1e-8, 1e-8])
+    grad = prob.backward(w)
+    fd_prob = adjoint.DesignProblem(build, particles, length, wl.footstep, precision=64, ckpt_every=0)
+    fd = np.zeros_like(theta)
+    for k in range(theta.size):
+        h = 1e-6 * max(abs(theta[k]), 1.0)
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd[k] = (float(np.sum(w * fd_prob.forward(tp))) - float(np.sum(w * fd_prob.forward(tm)))) / (2 * h)
+    rel = np.abs(grad - fd) / np.abs(fd).max()
+    return {"parameters": int(theta.size), "protons": int(final.shape[0]), "segments": int(prob.tracker.model.nseg),
+            "functional": "w . final state (seeded Gaussian w)", "fd_step": "1e-6 relative, FP64 forward",
+            "max_rel_err_vs_fd": float(rel.max()), "rel_err_per_parameter": [float(v) for v in rel],
+            "seconds": time.perf_counter() - t0}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -354,6 +394,25 @@ def main() -> int:
     log(f"[bench] bs kernel: {avg_ms * 1e3:.1f} us/launch, {achieved:.2f} TFLOP/s "
         f"({100 * roofline['frac']:.1f}% of FP32 vector peak)")
 
+    # field of the FP32 path against FP64 at the same recorded states (relative error over points
+    # whose field is at least 1 % of the largest, where a relative figure means something)
+    plan64 = model.plan(S.npt, 2)
+    _, partial64 = model.workspace(plan64)
+    b32, b64 = torch.empty_like(S.s0[:3]), torch.empty_like(S.s0[:3])
+    dns, bns = [], []
+    for i in range(rec.shape[0]):
+        pts = rec[i, :3].contiguous()
+        model.field_tensors(plan32, FP32, pts, partial32, b32)
+        model.field_tensors(plan64, 64, pts, partial64, b64)
+        dns.append((b32 - b64).norm(dim=0))
+        bns.append(b64.norm(dim=0))
+    dn, bn = torch.cat(dns), torch.cat(bns)
+    peak, dmax = float(bn.max()), float(dn.max())
+    keep = bn >= 0.01 * peak
+    rel = float((dn[keep] / bn[keep]).max()) if bool(keep.any()) else 0.0
+    field_err = {"max_rel": rel, "max_abs_T": dmax, "peak_T": peak, "points": int(dn.numel())}
+    log(f"[bench] fp32 field vs fp64: max rel {rel:.3g}, max abs {dmax:.3g} T")
+
     extras = None
     if not args.no_extras:
         extras = {}
@@ -382,10 +441,13 @@ def main() -> int:
         sh = A_.shape(FP32, 0, 0)
         fa = A_.forward_adjoint(sh, 1, 1)
         fa0 = A_.forward_adjoint(sh, 1, 1, ckpt_every=0)
-        extras["adjoint_section"] = dict(protons=A_.P, segments=int(A_.model.nseg), rk4_steps=A_.max_steps, **fa,
-                                         full_tape=fa0)
-        log(f"[bench] adjoint_section: fwd+adj {fa['ms_per_pass']:.1f} ms ({fa['forward_ms']:.1f} forward)")
+        A_P, A_seg, A_steps = A_.P, int(A_.model.nseg), A_.max_steps
         del A_
+        gc = gradient_check()
+        extras["adjoint_section"] = dict(protons=A_P, segments=A_seg, rk4_steps=A_steps, **fa,
+                                         full_tape=fa0, gradient_check=gc)
+        log(f"[bench] adjoint_section: fwd+adj {fa['ms_per_pass']:.1f} ms ({fa['forward_ms']:.1f} forward); "
+            f"gradient vs FD max rel {gc['max_rel_err_vs_fd']:.3g}")
 
     cpu = None
     if not args.no_cpu:
@@ -415,6 +477,7 @@ def main() -> int:
         "fp64_plain": fp64,
         "speedup_vs_fp64": (rate32 / fp64["particle_steps_per_s"]) if fp64 else None,
         "position_error_um_vs_fp64": err,
+        "field_error_vs_fp64": field_err,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extras": extras,

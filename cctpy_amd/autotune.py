@@ -49,7 +49,7 @@ def _store(key: str, val: dict) -> None:
 
 # Bumped whenever the kernels or the candidate set change, so that cached choices of an
 # older library are not reused.
-KERNEL_SET = 3
+KERNEL_SET = 4
 
 
 def _key(model: "engine.DeviceModel", npt: Sequence[int], precision: int) -> str:

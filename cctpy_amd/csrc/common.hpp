@@ -102,14 +102,14 @@ __device__ inline void load_points(const double* __restrict__ pts, int64_t ptot,
     }
 }
 
-// The 4 waves of the workgroup hold partial sums for the same point block;
+// The NW waves of the workgroup hold partial sums for the same point block;
 // they are added in wave order (deterministic) and written as one partial
 // per (point, component): element (i, c) at out[(i*3 + c) * nchunk].
-template <int PPL>
+template <int PPL, int NW = kWavesPerWG>
 __device__ inline void write_partial(double ax[PPL], double ay[PPL], double az[PPL], int wave, int lane,
                                      double* __restrict__ out, int nchunk) {
     constexpr int PB = kWave * PPL;
-    __shared__ double red[kWavesPerWG][3][PB];
+    __shared__ double red[NW][3][PB];
 #pragma unroll
     for (int j = 0; j < PPL; ++j) {
         red[wave][0][lane + kWave * j] = ax[j];
@@ -117,9 +117,12 @@ __device__ inline void write_partial(double ax[PPL], double ay[PPL], double az[P
         red[wave][2][lane + kWave * j] = az[j];
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < 3 * PB; k += kWG) {
+    for (int k = threadIdx.x; k < 3 * PB; k += kWave * NW) {
         const int c = k / PB, i = k % PB;
-        out[(int64_t)(i * 3 + c) * nchunk] = ((red[0][c][i] + red[1][c][i]) + red[2][c][i]) + red[3][c][i];
+        double v = red[0][c][i];
+#pragma unroll
+        for (int q = 1; q < NW; ++q) v += red[q][c][i];
+        out[(int64_t)(i * 3 + c) * nchunk] = v;
     }
 }
 
