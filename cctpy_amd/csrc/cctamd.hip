@@ -501,14 +501,11 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __rest
     const int tend = w.tile_start + w.tile_count;
     const float4* src = reinterpret_cast<const float4*>(seg32k);
     // missing tiles: zero current, |m|^2 = 1e6 keeps |q - m|^2 > 0
-    const float4 pad0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float4 pad2 = make_float4(0.0f, 1.0e6f, 0.0f, 0.0f);
+    const auto pad = [](int e) {
+        return e % 3 == 2 ? make_float4(0.0f, 1.0e6f, 0.0f, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    };
     for (int t0 = w.tile_start; t0 < tend; t0 += NS) {
-        for (int i = threadIdx.x; i < NS * ROW; i += kWG) {
-            const int s = i / ROW, e = i % ROW;
-            const int t = t0 + s;
-            lds[s][e] = t < tend ? src[(int64_t)t * ROW + e] : (e % 3 == 2 ? pad2 : pad0);
-        }
+        stage_tiles<NS, ROW>(lds, src, t0, tend, pad);
         __syncthreads();
         const bool real = t0 + stream < tend;
         const int t = real ? t0 + stream : w.tile_start;

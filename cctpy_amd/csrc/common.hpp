@@ -23,6 +23,35 @@ __host__ __device__ inline int pb_of(int ppl) { return ppl > 0 ? kWave * ppl : k
 inline bool valid_ppl(int ppl) { return ppl == 1 || ppl == 2 || ppl == 4 || ppl == -2 || ppl == -4; }
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Copies the rows of NS consecutive tiles (R float4 per tile, tiles t0 ..
+// t0+NS-1 of src) into dst[NS][R + 1] in LDS; rows of tiles at or past tend
+// get fill(e).  Each thread of the kWG-thread workgroup issues its global loads
+// in batches of 4 before the batch's LDS writes, so the loads of a batch overlap
+// instead of paying one memory round trip each.  The caller synchronises after.
+template <int NS, int R, typename Fill>
+__device__ __forceinline__ void stage_tiles(float4 (*dst)[R + 1], const float4* __restrict__ src, int t0, int tend,
+                                            Fill fill) {
+    constexpr int N = NS * R;
+    constexpr int IT = (N + kWG - 1) / kWG;
+    constexpr int BATCH = 4;
+#pragma unroll 1
+    for (int r0 = 0; r0 < IT; r0 += BATCH) {
+        float4 v[BATCH];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) {
+            const int i = (int)threadIdx.x + (r0 + b) * kWG;
+            const int s = i / R, e = i % R;
+            v[b] = fill(e);
+            if (r0 + b < IT && i < N && t0 + s < tend) v[b] = src[(int64_t)(t0 + s) * R + e];
+        }
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) {
+            const int i = (int)threadIdx.x + (r0 + b) * kWG;
+            if (r0 + b < IT && i < N) dst[i / R][i % R] = v[b];
+        }
+    }
+}
+
 // ------------------------------------------------------------ geometry
 // Mirrors CCT.__init__/phi_ksi_function/BipolarToroidalCoordinateSystem.convert
 // (final_code/packages/cct.py:96-164, 166-188, 213-224) with the same
