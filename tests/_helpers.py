@@ -7,7 +7,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def ensure_built():
-    from cctpy_amd import _build
+    # cctpy_amd/_build.py from its file: importing the package loads the library
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("cctpy_amd_build", os.path.join(ROOT, "cctpy_amd", "_build.py"))
+    _build = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(_build)
     if not os.path.exists(_build.LIB):
         _build.build()
     from oracle import c_oracle
