@@ -922,7 +922,11 @@ int64_t cctamd_plan(int64_t ndesign, const int64_t* design_ntile, const int64_t*
     for (int64_t d = 0; d < ndesign; ++d) {
         const int64_t nb = ceil_div(design_npt[d], pb);
         const int64_t T = design_ntile[d];
-        const int64_t nc = (nb > 0 && T > 0) ? ceil_div(T, per_item) : 0;
+        int64_t nc = (nb > 0 && T > 0) ? ceil_div(T, per_item) : 0;
+        // Workgroups go to the XCDs round-robin and item (b, c) is launched as
+        // n + b nc + c: with nc a multiple of the XCD count the items of one tile
+        // chunk (same segments, other point blocks) share an XCD and its L2.
+        if (nb > 1 && ceil_div(nc, kXcds) * kXcds <= T) nc = ceil_div(nc, kXcds) * kXcds;
         if (work) {
             if (!design_tile_off || !design_pt_off || !design_wbase || !design_nchunk) return CCTAMD_EINVAL;
             if (nc > 0x7fffffff) return CCTAMD_ERANGE;

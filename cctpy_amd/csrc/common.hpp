@@ -13,6 +13,7 @@ constexpr int kTile = CCTAMD_TILE;
 constexpr int kWave = 64;
 constexpr int kWavesPerWG = 4;
 constexpr int kWG = kWave * kWavesPerWG;
+constexpr int64_t kXcds = 8;  // accelerator complex dies of an MI355X (each with its own L2)
 
 inline int hip_status(hipError_t e) { return e == hipSuccess ? CCTAMD_OK : (int)e; }
 
