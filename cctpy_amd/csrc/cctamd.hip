@@ -324,26 +324,48 @@ struct KCRow {
     float4 a, b, c;  // kl.xyz, C.x | C.yz, -2m.xy | -2m.z, |m|^2, 0, 0
 };
 
+// acc += (lane S of this 16-lane row's src) * mul: one v_fmac_f32 whose first
+// source is read through the row-broadcast DPP control.  The compiler does not
+// fold a DPP move into a multiply-accumulate, so a separate move per broadcast
+// value would cost an instruction; the broadcast sources here are written only
+// by vector loads (no VALU write shortly before a DPP read, the one DPP hazard).
+template <int S>
+__device__ __forceinline__ void fmac_bcast(float& acc, float src, float mul) {
+    asm volatile("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                 : "+v"(acc)
+                 : "v"(src), "v"(mul), "i"(S));
+}
+
+// lane S of this row's src + x, as one v_add_f32 with the row-broadcast DPP source.
+template <int S>
+__device__ __forceinline__ float add_bcast(float src, float x) {
+    float r;
+    asm volatile("v_add_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                 : "=v"(r)
+                 : "v"(src), "v"(x), "i"(S));
+    return r;
+}
+
 template <int S>
 __device__ __forceinline__ void kc_row_steps(const KCRow& g, const float (&qx)[2], const float (&qy)[2],
                                              const float (&qz)[2], const float (&qq)[2], float (&kx)[2],
                                              float (&ky)[2], float (&kz)[2], float (&cx)[2], float (&cy)[2],
                                              float (&cz)[2]) {
-    const float klx = row_bcast<S>(g.a.x), kly = row_bcast<S>(g.a.y), klz = row_bcast<S>(g.a.z);
-    const float cmx = row_bcast<S>(g.a.w), cmy = row_bcast<S>(g.b.x), cmz = row_bcast<S>(g.b.y);
-    const float mx2 = row_bcast<S>(g.b.z), my2 = row_bcast<S>(g.b.w), mz2 = row_bcast<S>(g.c.x);
-    const float mm = row_bcast<S>(g.c.y);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-        const float r2 = __builtin_fmaf(qz[j], mz2, __builtin_fmaf(qy[j], my2, __builtin_fmaf(qx[j], mx2, qq[j] + mm)));
+        // ((|q|^2 + |m|^2) + qx (-2m.x)) + qy (-2m.y) + qz (-2m.z), as in k_bs_partial_f32k
+        float r2 = add_bcast<S>(g.c.y, qq[j]);
+        fmac_bcast<S>(r2, g.b.z, qx[j]);
+        fmac_bcast<S>(r2, g.b.w, qy[j]);
+        fmac_bcast<S>(r2, g.c.x, qz[j]);
         const float inv = __builtin_amdgcn_rsqf(r2);
         const float w3 = inv * inv * inv;
-        kx[j] += klx * w3;
-        ky[j] += kly * w3;
-        kz[j] += klz * w3;
-        cx[j] += cmx * w3;
-        cy[j] += cmy * w3;
-        cz[j] += cmz * w3;
+        fmac_bcast<S>(kx[j], g.a.x, w3);
+        fmac_bcast<S>(ky[j], g.a.y, w3);
+        fmac_bcast<S>(kz[j], g.a.z, w3);
+        fmac_bcast<S>(cx[j], g.a.w, w3);
+        fmac_bcast<S>(cy[j], g.b.x, w3);
+        fmac_bcast<S>(cz[j], g.b.y, w3);
     }
     if constexpr (S + 1 < 16) kc_row_steps<S + 1>(g, qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
 }
