@@ -310,41 +310,13 @@ __global__ __launch_bounds__(kWave * NW) void k_bs_partial_f32k(const float* __r
 // two points per lane that walks its own tiles.  Lane i of a row holds the K/C
 // rows of segments i and i + 16 of the row's tile in registers (coalesced
 // vector loads), and segment k reaches the whole row through the row-broadcast
-// DPP move (lane k of each row to all 16 lanes): 10 moves per segment shared by
-// 2 points, no LDS traffic and no barriers.  Registers bound its occupancy: 80
-// VGPRs (6 waves per SIMD, no spills) with the tile loaded at the top of each
-// round measured 4 % faster than prefetching the next tile at 101 (4 waves).
-template <int S>
-__device__ __forceinline__ float row_bcast(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + S, 0xf, 0xf,
-                                                                  true));
-}
-
+// DPP control (lane k of each row to all 16 lanes) on the instruction that
+// consumes the value, no LDS traffic and no barriers.  Registers bound its
+// occupancy: with the tile loaded at the top of each round and a 6-wave hint it
+// measured faster than prefetching the next tile (fewer waves per SIMD).
 struct KCRow {
     float4 a, b, c;  // kl.xyz, C.x | C.yz, -2m.xy | -2m.z, |m|^2, 0, 0
 };
-
-// acc += (lane S of this 16-lane row's src) * mul: one v_fmac_f32 whose first
-// source is read through the row-broadcast DPP control.  The compiler does not
-// fold a DPP move into a multiply-accumulate, so a separate move per broadcast
-// value would cost an instruction; the broadcast sources here are written only
-// by vector loads (no VALU write shortly before a DPP read, the one DPP hazard).
-template <int S>
-__device__ __forceinline__ void fmac_bcast(float& acc, float src, float mul) {
-    asm volatile("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
-                 : "+v"(acc)
-                 : "v"(src), "v"(mul), "i"(S));
-}
-
-// lane S of this row's src + x, as one v_add_f32 with the row-broadcast DPP source.
-template <int S>
-__device__ __forceinline__ float add_bcast(float src, float x) {
-    float r;
-    asm volatile("v_add_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
-                 : "=v"(r)
-                 : "v"(src), "v"(x), "i"(S));
-    return r;
-}
 
 template <int S>
 __device__ __forceinline__ void kc_row_steps(const KCRow& g, const float (&qx)[2], const float (&qy)[2],

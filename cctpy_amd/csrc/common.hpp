@@ -24,6 +24,40 @@ __host__ __device__ inline int pb_of(int ppl) { return ppl > 0 ? kWave * ppl : k
 inline bool valid_ppl(int ppl) { return ppl == 1 || ppl == 2 || ppl == 4 || ppl == -2 || ppl == -4; }
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// ---------------------------------------------------- row broadcasts
+// Value of lane S of this 16-lane row, in every lane of the row (DPP
+// row_newbcast, a move).
+template <int S>
+__device__ __forceinline__ float row_bcast(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + S, 0xf, 0xf,
+                                                                  true));
+}
+
+// The helpers below put the broadcast on the instruction that consumes it: VOP2
+// forms whose first source is read through the row_newbcast DPP control.  The
+// compiler does not fold a DPP move into a multiply-accumulate, so a separate
+// move per broadcast value would cost an instruction.  Their broadcast sources
+// must not be written by a VALU instruction shortly before (the DPP hazard the
+// compiler does not check in inline assembly; tests/test_dpp_hazards.py scans
+// the built kernels).
+// acc += (lane S of this 16-lane row's src) * mul.
+template <int S>
+__device__ __forceinline__ void fmac_bcast(float& acc, float src, float mul) {
+    asm volatile("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                 : "+v"(acc)
+                 : "v"(src), "v"(mul), "i"(S));
+}
+
+// lane S of this row's src + x.
+template <int S>
+__device__ __forceinline__ float add_bcast(float src, float x) {
+    float r;
+    asm volatile("v_add_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                 : "=v"(r)
+                 : "v"(src), "v"(x), "i"(S));
+    return r;
+}
+
 // Copies the rows of NS consecutive tiles (R float4 per tile, tiles t0 ..
 // t0+NS-1 of src) into dst[NS][R + 1] in LDS; rows of tiles at or past tend
 // get fill(e).  Each thread of the kWG-thread workgroup issues its global loads
