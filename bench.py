@@ -216,14 +216,18 @@ class Setup:
         ms = self.timed(one_pass, passes, warmup) / passes
         return ms, self.st.cpu().numpy().T.copy(), plan, partial
 
-    def forward_adjoint(self, shape: dict, passes: int, warmup: int, ckpt_every: int = CKPT_EVERY):
+    def forward_adjoint(self, shape: dict, passes: int, warmup: int, ckpt_every: int = CKPT_EVERY,
+                        tune: bool = True):
         """ms per FP32 forward (with checkpoints every ckpt_every steps, or the full tape with 0)
-        + objective + adjoint pass, and the split."""
+        + objective + adjoint pass, and the split.  With ``tune`` the block shape is the one
+        with the fastest short forward + adjoint pass; otherwise ``shape``'s."""
         import torch
         from cctpy_amd.adjoint import AdjointTracker
         from cctpy_amd.beam import BeamGroups
         from cctpy_amd.particles import ParticleFactory
         tr = AdjointTracker(self.bls, precision=FP32, ckpt_every=ckpt_every, ppl=shape["ppl"])
+        if tune:  # forward + backward decide the block shape, not the forward kernel alone
+            tr.tune(self.npt, self.s0, self.kq, self.dt, self.ns_t)
         tr.prepare(self.npt, self.kq, self.dt, self.ns_t, self.max_steps)
         ideals, frames = [], []
         for i, (bl, beam) in enumerate(zip(self.bls, self.wl.beams)):
@@ -323,7 +327,7 @@ def main() -> int:
     if args.mode == "adjoint":
         A_ = Setup(W.by_name(args.workload), align_cct=True, max_steps=args.max_steps)
         fa = A_.forward_adjoint(A_.shape(FP32, args.ppl, args.items_per_cu), args.steps, args.warmup,
-                                ckpt_every=args.ckpt_every)
+                                ckpt_every=args.ckpt_every, tune=args.ppl == 0)
         print(json.dumps(dict(workload=A_.wl.name, protons=A_.P, segments=int(A_.model.nseg),
                               rk4_steps=A_.max_steps, **fa)), flush=True)
         return 0

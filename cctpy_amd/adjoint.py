@@ -74,6 +74,34 @@ class AdjointTracker:
         self.bs_pts = torch.zeros((3, P), dtype=torch.float64, device=dev)
         self.bs_wts = torch.zeros((3, P), dtype=torch.float64, device=dev)
 
+    def tune(self, design_npt: Sequence[int], state: torch.Tensor, kq: torch.Tensor, dt: torch.Tensor,
+             nsteps: torch.Tensor, candidates: Sequence[int] = (1, 2, -2), probe_steps: int = 8) -> int:
+        """Sets (and returns) the points-per-lane code whose forward + adjoint pass over the first
+        ``probe_steps`` steps is fastest.  The backward kernel dominates the adjoint and prefers
+        other block shapes than the forward alone (2 points per lane against 1 on a 256-proton
+        section), so the shape is chosen on both; call prepare() afterwards."""
+        ns = torch.clamp(nsteps, max=probe_steps)
+        best, best_ms = self.ppl, math.inf
+        for ppl in candidates:
+            self.ppl = ppl
+            self.prepare(design_npt, kq, dt, ns, probe_steps)
+            seed = torch.zeros((6, self.P), dtype=torch.float64, device=state.device)
+            seed[:3] = 1.0
+            times = []
+            for _ in range(2):  # the first pass warms up
+                st = state.clone()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                self.forward_tensors(st)
+                self.backward_tensors(seed.clone())
+                e1.record()
+                torch.cuda.synchronize()
+                times.append(e0.elapsed_time(e1))
+            if times[-1] < best_ms:
+                best, best_ms = ppl, times[-1]
+        self.ppl = best
+        return best
+
     def forward_tensors(self, state: torch.Tensor) -> None:
         """In-place tracking of device state (6, P), keeping checkpoints every K steps (or, with
         K = 0, the tape of every step)."""

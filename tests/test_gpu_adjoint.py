@@ -218,3 +218,36 @@ def test_adjoint_full_tape_matches_checkpoints():
         assert np.array_equal(out[0][0], out[1][0])
         assert np.array_equal(out[0][1], out[1][1])
         assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_adjoint_tune_picks_a_candidate_and_keeps_results():
+    """tune() times short forward + adjoint passes, sets one of the candidate block shapes,
+    and a tracker prepared after it gives the bits of a fresh tracker with that shape."""
+    from cctpy_amd import engine
+    wl = W.population(n_designs=2, n_particles=16, seed=6)
+    bls = [W.build_beamline(d) for d in wl.designs]
+    ps, npt, lengths = [], [], []
+    for i, (bl, beam) in enumerate(zip(bls, wl.beams)):
+        q = W.beam_particles(beam, bl)
+        ps += q
+        npt.append(len(q))
+        lengths += [0.4] * len(q)
+    state = np.array([p.position.to_list() + p.velocity.to_list() for p in ps])
+    kq = np.array([p.e / p.relativistic_mass for p in ps])
+    sd = [engine.rk4_steps(L, 2e-3, p.speed) for L, p in zip(lengths, ps)]
+    ns = np.array([s for s, _ in sd], np.int64)
+    dt = np.array([d for _, d in sd])
+    dev = engine.device()
+    s0 = torch.from_numpy(np.ascontiguousarray(state.T)).to(dev)
+    kq_t, dt_t, ns_t = (torch.from_numpy(a).to(dev) for a in (kq, dt, ns))
+    w = np.random.default_rng(5).normal(size=(len(ps), 6)) * np.array([1.0, 1.0, 1.0, 1e-8, 1e-8, 1e-8])
+    tr = adjoint.AdjointTracker(bls, precision=33, ckpt_every=16, ppl=2)
+    best = tr.tune(npt, s0, kq_t, dt_t, ns_t, candidates=(1, 2, -2), probe_steps=6)
+    assert best in (1, 2, -2) and tr.ppl == best
+    out = []
+    for t in (tr, adjoint.AdjointTracker(bls, precision=33, ckpt_every=16, ppl=best)):
+        final = t.forward(state, kq, dt, ns, npt)
+        a0, cg, qg = t.backward(w)
+        out.append((final, a0, cg))
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
