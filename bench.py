@@ -470,7 +470,9 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 Biot-Savart terms; f64 tile sums and RK4 state",
+        "dtype": "f32",
+        "dtype_detail": "f32 Biot-Savart pair terms and 32-segment tile sums; f64 tile origins, sums across tiles "
+                        "and RK4 state",
         "data": "synthetic",
         "config": {"workload": wl.name, "designs": len(S.bls), "segments": int(model.nseg), "protons": S.P,
                    "rk4_steps": S.max_steps, "steps_capped": args.max_steps > 0,
