@@ -1,6 +1,7 @@
 """CPU tests: C ABI exports, host planner, and host-side geometry against the oracle."""
 import ctypes
 import math
+import os
 import re
 
 import numpy as np
@@ -175,3 +176,16 @@ def test_population_adapter_host_parts():
     except ImportError:
         with pytest.raises(ImportError):
             optim.geatpy_problem()
+
+
+def test_integration_binding_matches_the_loader():
+    """The ctypes binding shown in INTEGRATION.md declares the argument types the package's own
+    loader declares for the same entry points."""
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    native = open(os.path.join(ROOT, "cctpy_amd", "_native.py")).read()
+    found = re.findall(r"lib\.(cctamd_\w+)\.argtypes = (\[[^\]]*\])", doc)
+    assert len(found) >= 4
+    for name, args in found:
+        m = re.search(r"lib\." + name + r"\.argtypes = (\[[^\]]*\])", native)
+        assert m, name
+        assert re.sub(r"\s", "", args) == re.sub(r"\s", "", m.group(1)), name
