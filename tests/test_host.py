@@ -70,17 +70,19 @@ def test_plan_covers_every_point_tile_pair_once(ppl):
 
 
 def test_plan_chunks_of_one_tile_range_share_an_xcd():
-    """368,640 segments (11,520 tiles) x 512 points in 128-point blocks at ~2,048 items: the
-    chunk count is rounded up to a multiple of the 8 XCDs (501 -> 504), so the items of one
-    chunk (launched as wbase + b nchunk + c, dispatched round-robin) land on one XCD."""
-    work, wbase, nch, toff, poff = _plan([11520], [512], 2, target=2048)
-    assert nch[0] == 504
+    """A design's chunk count is rounded up to a multiple of the 8 XCDs (1,000 tiles x 512
+    points in 128-point blocks at 10 tiles per item: 100 -> 104 chunks), so that the items of one
+    chunk (launched as wbase + b nchunk + c, dispatched round-robin) land on one XCD; for the
+    split-wave codes the tiles per item are a multiple of the 16 tile streams."""
+    work, wbase, nch, toff, poff = _plan([11520], [64], -2, target=600)  # 39 -> 48 tiles per item
+    assert nch[0] == 240 and all(w[4] == 48 for w in work)
+    work, wbase, nch, toff, poff = _plan([1000], [512], 2, target=400)  # 10 tiles per item: 100 -> 104
+    assert nch[0] == 104
     for c in range(nch[0]):
         xcd = {(wbase[0] + b * nch[0] + c) % 8 for b in range(4)}
         assert len(xcd) == 1
-    tiles = sorted((w[3], w[4]) for w in work[:nch[0]])
-    assert sum(n for _, n in tiles) == 11520 and min(n for _, n in tiles) > 0
-
+    tiles = [w[4] for w in work[:nch[0]]]
+    assert sum(tiles) == 1000 and min(tiles) > 0
 def test_plan_rejects_bad_arguments():
     z = np.zeros(1, np.int64)
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
