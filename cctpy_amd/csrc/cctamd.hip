@@ -7,9 +7,12 @@
 //   k_pack_fp32         FP64 segments  -> FP32 tiles re-centred on the tile
 //                                         origin (+ K/C rows)
 //   k_bs_partial_*<N>   split-K Biot-Savart: one workgroup = one point block x
-//                       one chunk of tiles; the 4 waves share the block and
-//                       interleave tiles; segments are wave-uniform (scalar
-//                       loads into SGPRs), points live in VGPRs (N per lane)
+//                       one chunk of tiles; its waves (8 in the scalar-load
+//                       kernels) share the block and interleave tiles; segments
+//                       are wave-uniform (scalar loads into SGPRs), points live
+//                       in VGPRs (N per lane)
+//   k_bs_partial_f32k_row  32-point blocks: 16-lane rows walk their own tiles,
+//                       segments broadcast within a row by DPP
 //   k_field_reduce      fixed-order sum of the chunk partials + QS + uniform
 //   k_rk4_stage<S>      same reduction, then RK4 stage S of each particle
 //
