@@ -181,7 +181,7 @@ def test_population_adapter_host_parts():
 
 
 def test_integration_binding_matches_the_loader():
-    """The ctypes binding shown in INTEGRATION.md declares the argument types the package's own
+    """The ctypes binding shown in the integration guide declares the argument types the package's own
     loader declares for the same entry points."""
     doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     native = open(os.path.join(ROOT, "cctpy_amd", "_native.py")).read()
