@@ -700,8 +700,11 @@ __global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __rest
     double* vs = scratch + 3 * ptot;
     double* kp = scratch + 6 * ptot;
     double* kv = scratch + 9 * ptot;
-    // every load that does not depend on the partial sums is issued first
-    const bool active = step < nsteps[g];
+    // The partial-sum slot first (the chunk loads depend on it), then every other load; the
+    // finished-particle test comes after the chunk sum, so that the compiler does not wait for
+    // nsteps before issuing the rest (finished particles keep their state).
+    const PointSlot sl = load_slot(scratch, ptot, g, partial);
+    const int64_t ns = nsteps[g];
     const double k = kq[g];
     const double h = dt[g];
     double p[3], v[3], p0[3], v0[3], sp[3], sv[3];
@@ -716,10 +719,9 @@ __global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __rest
             sv[c] = kv[o];
         }
     }
-    const PointSlot sl = load_slot(scratch, ptot, g, partial);
-    if (!active) return;  // finished particles keep their state
     double b[3];
     chunk_sum(sl, b);
+    if (step >= ns) return;
     if ((threadIdx.x & (kWave - 1)) != 0) return;
     analytic_fields(sl.design, qs, design_qs_off, bconst, p, b);
     if (tape) {  // stage point, velocity and field for the adjoint
