@@ -254,6 +254,46 @@ class Setup:
                 "grad_finite": bool(torch.isfinite(tr.tgrad).all().item())}
 
 
+def field_along_arc(n_points: int = 1000, reps: int = 20) -> dict:
+    """Field of the single-CCT workload (23,040 segments) at points spaced evenly along the ideal
+    arc in the median plane: FP64 and FP32 paths timed (HIP events, inputs on the device), the
+    FP32 relative error against FP64 over points whose field is at least 1 % of the largest."""
+    import numpy as np
+    import torch
+    from cctpy_amd import engine
+    from cctpy_amd import workloads as W
+    from cctpy_amd.magnets import device_parts
+    wl = W.single_cct(n_particles=1)
+    bl = W.build_beamline(wl.designs[0])
+    model = engine.DeviceModel([device_parts(bl).spec()], fp32=True)
+    s0 = wl.beams[0].s0
+    pts = np.array([[q.x, q.y, 0.0] for q in (bl.point_at(float(v))
+                                                 for v in np.linspace(s0, s0 + wl.length, n_points))])
+    dev = engine.device()
+    soa = torch.from_numpy(np.ascontiguousarray(pts.T)).to(dev)
+    out, res = {}, {}
+    for prec, ppl in ((64, 1), (FP32, 1)):
+        plan = model.plan([n_points], ppl)
+        _, partial = model.workspace(plan)
+        b = torch.empty_like(soa)
+        model.field_tensors(plan, prec, soa, partial, b)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            model.field_tensors(plan, prec, soa, partial, b)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[prec] = b.clone()
+        out["fp64" if prec == 64 else "fp32"] = {"ms": ms, "pair_evals_per_s": n_points * model.nseg / (ms / 1e3)}
+    dn, bn = (res[FP32] - res[64]).norm(dim=0), res[64].norm(dim=0)
+    keep = bn >= 0.01 * bn.max()
+    out.update({"points": n_points, "segments": int(model.nseg), "launches_per_evaluation": 2,
+                "fp32_max_rel_err": float((dn[keep] / bn[keep]).max()), "peak_T": float(bn.max())})
+    return out
+
+
 def gradient_check(n_particles: int = 16) -> dict:
     """dJ/dtheta by the adjoint (FP32 Biot-Savart, full tape) for a linear functional of the final
     states, over the 10-parameter design vector of the adjoint_section workload (full geometry and
@@ -447,6 +487,9 @@ def main() -> int:
         fa0 = A_.forward_adjoint(sh, 1, 1, ckpt_every=0)
         A_P, A_seg, A_steps = A_.P, int(A_.model.nseg), A_.max_steps
         del A_
+        extras["single_cct_field_along_arc"] = field_along_arc()
+        log(f"[bench] single_cct field at 1000 arc points: fp64 {extras['single_cct_field_along_arc']['fp64']['ms']:.3f} ms, "
+            f"fp32 {extras['single_cct_field_along_arc']['fp32']['ms']:.3f} ms")
         gc = gradient_check()
         extras["adjoint_section"] = dict(protons=A_P, segments=A_seg, rk4_steps=A_steps, **fa,
                                          full_tape=fa0, gradient_check=gc)
