@@ -257,9 +257,10 @@ __global__ __launch_bounds__(kWG) void k_bs_backward_f32(
 // splits into G lane groups that hold the same points and walk different tiles
 // (seg32 rows and tangents staged in LDS, read as broadcasts); the per-segment
 // sums reduce within a group, and the point adjoints across groups at the end.
-// With G = 2 the occupancy hint of 5 waves per SIMD (92 VGPRs, no spills)
-// measured 10 % faster on the 64-design population than 4 waves at 111 VGPRs;
-// the same hint on k_bs_backward_f32<2> made it 9 % slower, so it has none.
+// It serves 16-point blocks (G = 4); 32-point blocks use k_bs_backward_f32_grp2<4>,
+// 3 % faster on the 64-design population than this kernel with G = 2 (which ran
+// 10 % faster with a 5-wave occupancy hint than without; the same hint made
+// k_bs_backward_f32<2> 9 % slower, so that one has none).
 template <int G>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(G == 2 ? 5 : 4))) void k_bs_backward_f32_grp(
     const float* __restrict__ seg32, const double* __restrict__ tile_org, const float* __restrict__ seg_tan,
@@ -349,6 +350,154 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(G == 2 ? 5 
         red[wave][0][lane] = ax;
         red[wave][1][lane] = ay;
         red[wave][2][lane] = az;
+    }
+    __syncthreads();
+    double* out = partial + partial_base(w, PB);
+    for (int k = threadIdx.x; k < 3 * PB; k += kWG) {
+        const int c = k / PB, i = k % PB;
+        out[(int64_t)(i * 3 + c) * w.nchunk] = ((red[0][c][i] + red[1][c][i]) + red[2][c][i]) + red[3][c][i];
+    }
+}
+
+// Split-wave form with two points per lane: G lane groups of 64 / G lanes, each
+// lane holding points gl and gl + 64 / G of the block (32-point blocks for G = 4),
+// so that the tangent contraction of a segment and its LDS reads are shared by
+// two points (~50 VALU instructions per pair against ~58).  Its 16 tile streams
+// take ~37 KB of LDS per workgroup, which holds it at 4 waves per SIMD.
+template <int G>
+__global__ __launch_bounds__(kWG) void k_bs_backward_f32_grp2(
+    const float* __restrict__ seg32, const double* __restrict__ tile_org, const float* __restrict__ seg_tan,
+    const int32_t* __restrict__ tile_cct, const double* __restrict__ cct_coef, const Work* __restrict__ work,
+    const double* __restrict__ pts, const double* __restrict__ wts, int64_t ptot, int64_t ntile_total,
+    double* __restrict__ partial, double* __restrict__ tgrad) {
+    constexpr int LG = kWave / G;        // lanes per group
+    constexpr int PB = 2 * LG;           // points per block
+    constexpr int NS = kWavesPerWG * G;  // tile streams per workgroup
+    constexpr int SROW = kTile * 2;      // float4 per tile of seg32
+    constexpr int TROW = kTanRows * 2;   // float4 per tile of tangents
+    __shared__ float4 lseg[NS][SROW + 1];
+    __shared__ float4 ltan[NS][TROW + 1];
+    __shared__ double red[kWavesPerWG][3][PB];
+    const Work w = work[blockIdx.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int gl = lane % LG;
+    const int stream = wave * G + lane / LG;
+    double px[2], py[2], pz[2], ax[2], ay[2], az[2];
+    float bx[2], by[2], bz[2], hbx[2], hby[2], hbz[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int i = gl + LG * j;
+        const bool has = i < w.pt_count;
+        const int64_t gp = (int64_t)w.pt_start + (has ? i : 0);
+        px[j] = pts[gp];
+        py[j] = pts[ptot + gp];
+        pz[j] = pts[2 * ptot + gp];
+        bx[j] = has ? (float)wts[gp] : 0.0f;  // surplus lanes carry no weight
+        by[j] = has ? (float)wts[ptot + gp] : 0.0f;
+        bz[j] = has ? (float)wts[2 * ptot + gp] : 0.0f;
+        hbx[j] = 0.5f * bx[j];
+        hby[j] = 0.5f * by[j];
+        hbz[j] = 0.5f * bz[j];
+        ax[j] = ay[j] = az[j] = 0.0;
+    }
+    const int tend = w.tile_start + w.tile_count;
+    const float4* s4 = reinterpret_cast<const float4*>(seg32);
+    const float4* t4 = reinterpret_cast<const float4*>(seg_tan);
+    for (int t0 = w.tile_start; t0 < tend; t0 += NS) {
+        const auto zero = [](int) { return make_float4(0.0f, 0.0f, 0.0f, 0.0f); };
+        stage_tiles<NS, SROW>(lseg, s4, t0, tend, zero);
+        stage_tiles<NS, TROW>(ltan, t4, t0, tend, zero);
+        __syncthreads();
+        const int t = t0 + stream;
+        if (t < tend) {  // uniform within a lane group
+            const int cct = tile_cct[t];
+            const float coef = cct >= 0 ? (float)cct_coef[cct] : 0.0f;
+            float qx[2], qy[2], qz[2], sx[2], sy[2], sz[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                qx[j] = (float)(px[j] - tile_org[4 * t]);
+                qy[j] = (float)(py[j] - tile_org[4 * t + 1]);
+                qz[j] = (float)(pz[j] - tile_org[4 * t + 2]);
+                sx[j] = sy[j] = sz[j] = 0.0f;
+            }
+            float cur = 0.0f;
+            TanAcc ta{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            const float4* sr = lseg[stream];
+            const float4* tr = ltan[stream];
+#pragma unroll 2
+            for (int k = 0; k < kTile; ++k) {
+                const float4 kl = sr[2 * k], m = sr[2 * k + 1];
+                float gkx, gky, gkz, ghx, ghy, ghz;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const BwdTerm b = bwd_term(qx[j], qy[j], qz[j], bx[j], by[j], bz[j], hbx[j], hby[j], hbz[j], kl, m);
+                    sx[j] += b.hx;
+                    sy[j] += b.hy;
+                    sz[j] += b.hz;
+                    cur += b.e;
+                    if (j == 0) {
+                        gkx = b.w * b.cx;
+                        gky = b.w * b.cy;
+                        gkz = b.w * b.cz;
+                        ghx = b.hx;
+                        ghy = b.hy;
+                        ghz = b.hz;
+                    } else {
+                        gkx = __builtin_fmaf(b.w, b.cx, gkx);
+                        gky = __builtin_fmaf(b.w, b.cy, gky);
+                        gkz = __builtin_fmaf(b.w, b.cz, gkz);
+                        ghx += b.hx;
+                        ghy += b.hy;
+                        ghz += b.hz;
+                    }
+                }
+                const float4 e0 = tr[2 * k], e1 = tr[2 * k + 1];
+                tan_step(ta, coef * e1.w, gkx, gky, gkz, ghx, ghy, ghz, e0, e1);
+            }
+            tan_end(ta, tr[2 * kTile], tr[2 * kTile + 1]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {  // s holds u / 2
+                ax[j] += 2.0 * (double)sx[j];
+                ay[j] += 2.0 * (double)sy[j];
+                az[j] += 2.0 * (double)sz[j];
+            }
+            float g0 = ta.g0, g1 = ta.g1, g2 = ta.g2, g3 = ta.g3;
+#pragma unroll
+            for (int off = LG / 2; off > 0; off >>= 1) {  // sum over the group's lanes
+                cur += __shfl_xor(cur, off);
+                g0 += __shfl_xor(g0, off);
+                g1 += __shfl_xor(g1, off);
+                g2 += __shfl_xor(g2, off);
+                g3 += __shfl_xor(g3, off);
+            }
+            if (gl == 0 && cct >= 0) {
+                double* tg = tgrad + ((int64_t)w.block * ntile_total + t) * kTileGrad;
+                tg[0] += (double)cur;
+                tg[1] += (double)g0;
+                tg[2] += (double)g1;
+                tg[3] += (double)g2;
+                tg[4] += (double)g3;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+        for (int off = LG; off < kWave; off <<= 1) {  // add the lane groups (fixed order)
+            ax[j] += __shfl_xor(ax[j], off);
+            ay[j] += __shfl_xor(ay[j], off);
+            az[j] += __shfl_xor(az[j], off);
+        }
+    }
+    if (lane < LG) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            red[wave][0][lane + LG * j] = ax[j];
+            red[wave][1][lane + LG * j] = ay[j];
+            red[wave][2][lane + LG * j] = az[j];
+        }
     }
     __syncthreads();
     double* out = partial + partial_base(w, PB);
@@ -525,8 +674,8 @@ hipError_t launch_bwd_any(int ppl, const float* seg32, const double* tile_org, c
                           double* tgrad, hipStream_t st) {
     if (ppl < 0) {
         if (nwork == 0) return hipSuccess;
-        if (ppl == -2)
-            hipLaunchKernelGGL(k_bs_backward_f32_grp<2>, dim3((unsigned)nwork), dim3(kWG), 0, st, seg32, tile_org,
+        if (ppl == -2)  // 32-point blocks: 4 groups of 16 lanes, 2 points per lane
+            hipLaunchKernelGGL(k_bs_backward_f32_grp2<4>, dim3((unsigned)nwork), dim3(kWG), 0, st, seg32, tile_org,
                                seg_tan, tile_cct, cct_coef, work, pts, wts, ptot, ntile, partial, tgrad);
         else
             hipLaunchKernelGGL(k_bs_backward_f32_grp<4>, dim3((unsigned)nwork), dim3(kWG), 0, st, seg32, tile_org,
