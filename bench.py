@@ -181,7 +181,7 @@ class Setup:
             if not hasattr(self, "_samples"):
                 self._samples = self.track_samples()
             return autotune.tune(self.model, self.npt, prec, self._samples)
-        if prec != FP32 and ppl < 0:
+        if prec == 32 and ppl < 0:
             ppl = 2
         return {"ppl": ppl, "target_items": items_per_cu * engine.cu_count() or None}
 

@@ -49,7 +49,7 @@ def _store(key: str, val: dict) -> None:
 
 # Bumped whenever the kernels or the candidate set change, so that cached choices of an
 # older library are not reused.
-KERNEL_SET = 4
+KERNEL_SET = 5
 
 
 def _key(model: "engine.DeviceModel", npt: Sequence[int], precision: int) -> str:
@@ -75,7 +75,7 @@ def tune(model: "engine.DeviceModel", npt: Sequence[int], precision: int, points
             return disk
     sets = list(points) if isinstance(points, (list, tuple)) else [points]
     cus = engine.cu_count()
-    ppls = PPL + ((-2, -4) if precision == 33 else ())
+    ppls = PPL + ((-2, -4) if precision != 32 else ())
     cost: Dict[tuple, float] = {}
     for _ in range(max(1, rounds)):
         for ppl in ppls:

@@ -637,6 +637,88 @@ __global__ __launch_bounds__(kWave * NW) void k_bs_partial_f64(const double* __r
     write_partial<PPL, NW>(ax, ay, az, wave, lane, partial + partial_base(w, PB), w.nchunk);
 }
 
+// FP64 split-wave form of k_bs_partial_f64 for designs with few particles
+// (points-per-lane codes -2 / -4): G lane groups of 64 / G lanes hold the same
+// PL points per lane and walk different tiles, FP64 segment rows staged in LDS
+// and read as broadcasts, so no lane idles where a 64-point block would be
+// half empty.  Same pair term and summation order per point as k_bs_partial_f64.
+template <int G, int PL>
+__global__ __launch_bounds__(kWG) void k_bs_partial_f64_grp(const double* __restrict__ seg64,
+                                                             const Work* __restrict__ work,
+                                                             const double* __restrict__ pts, int64_t ptot,
+                                                             double* __restrict__ partial) {
+    constexpr int LG = kWave / G;         // lanes per group
+    constexpr int PB = LG * PL;           // points per block
+    constexpr int NS = kWavesPerWG * G;   // tile streams per workgroup
+    constexpr int ROW = kTile * 4;        // float4 (one double2 each) per tile
+    __shared__ float4 lds[NS][ROW + 1];
+    __shared__ double red[kWavesPerWG][3][PB];
+    const Work w = work[blockIdx.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int gl = lane % LG;
+    const int stream = wave * G + lane / LG;
+    double px[PL], py[PL], pz[PL], ax[PL], ay[PL], az[PL];
+#pragma unroll
+    for (int j = 0; j < PL; ++j) {
+        int idx = gl + LG * j;
+        idx = idx < w.pt_count ? idx : 0;  // surplus lanes duplicate point 0, results discarded
+        const int64_t gp = (int64_t)w.pt_start + idx;
+        px[j] = pts[gp];
+        py[j] = pts[ptot + gp];
+        pz[j] = pts[2 * ptot + gp];
+        ax[j] = ay[j] = az[j] = 0.0;
+    }
+    const int tend = w.tile_start + w.tile_count;
+    const float4* src = reinterpret_cast<const float4*>(seg64);
+    const auto zero = [](int) { return make_float4(0.0f, 0.0f, 0.0f, 0.0f); };  // rows of absent tiles: not read
+    for (int t0 = w.tile_start; t0 < tend; t0 += NS) {
+        stage_tiles<NS, ROW>(lds, src, t0, tend, zero);
+        __syncthreads();
+        if (t0 + stream < tend) {  // uniform within a lane group
+            const double2* r = reinterpret_cast<const double2*>(lds[stream]);
+#pragma unroll 4
+            for (int k = 0; k < kTile; ++k) {
+                const double2 k01 = r[4 * k], k2_ = r[4 * k + 1], m01 = r[4 * k + 2], m2_ = r[4 * k + 3];
+#pragma unroll
+                for (int j = 0; j < PL; ++j) {
+                    const double rx = px[j] - m01.x, ry = py[j] - m01.y, rz = pz[j] - m2_.x;
+                    const double r2 = rx * rx + ry * ry + rz * rz;
+                    const double inv = rsqrt_f64(r2);
+                    const double inv3 = inv * inv * inv;
+                    ax[j] += (k01.y * rz - k2_.x * ry) * inv3;
+                    ay[j] += (k2_.x * rx - k01.x * rz) * inv3;
+                    az[j] += (k01.x * ry - k01.y * rx) * inv3;
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < PL; ++j) {
+#pragma unroll
+        for (int off = LG; off < kWave; off <<= 1) {  // add the lane groups (fixed order)
+            ax[j] += __shfl_xor(ax[j], off);
+            ay[j] += __shfl_xor(ay[j], off);
+            az[j] += __shfl_xor(az[j], off);
+        }
+    }
+    if (lane < LG) {
+#pragma unroll
+        for (int j = 0; j < PL; ++j) {
+            red[wave][0][lane + LG * j] = ax[j];
+            red[wave][1][lane + LG * j] = ay[j];
+            red[wave][2][lane + LG * j] = az[j];
+        }
+    }
+    __syncthreads();
+    double* out = partial + partial_base(w, PB);
+    for (int k = threadIdx.x; k < 3 * PB; k += kWG) {
+        const int c = k / PB, i = k % PB;
+        out[(int64_t)(i * 3 + c) * w.nchunk] = ((red[0][c][i] + red[1][c][i]) + red[2][c][i]) + red[3][c][i];
+    }
+}
+
 // ------------------------------------------------ reduction + QS + RK4
 __global__ __launch_bounds__(kWG) void k_field_reduce(int ppl, const double* __restrict__ partial,
                                                       const int64_t* __restrict__ design_wbase,
@@ -793,7 +875,7 @@ hipError_t launch_bs(int precision, const void* seg, const double* tile_org, con
 bool valid_common(int precision, int ppl, const void* seg, const double* tile_org, int64_t nwork,
                   int64_t ndesign) {
     if (precision != 32 && precision != 33 && precision != 64) return false;
-    if (!valid_ppl(ppl) || (ppl < 0 && precision != 33)) return false;
+    if (!valid_ppl(ppl) || (ppl < 0 && precision == 32)) return false;
     if (nwork < 0 || ndesign <= 0 || nwork > 0x7fffffff) return false;
     if (nwork > 0 && seg == nullptr) return false;
     if (nwork > 0 && precision != 64 && tile_org == nullptr) return false;
@@ -804,6 +886,15 @@ hipError_t launch_bs_any(int precision, int ppl, const void* seg, const double* 
                          int64_t nwork, const double* pts, int64_t ptot, double* partial, hipStream_t st) {
     if (ppl < 0) {
         if (nwork == 0) return hipSuccess;
+        if (precision == 64) {
+            if (ppl == -2)
+                hipLaunchKernelGGL((k_bs_partial_f64_grp<4, 2>), dim3((unsigned)nwork), dim3(kWG), 0, st,
+                                   (const double*)seg, work, pts, ptot, partial);
+            else
+                hipLaunchKernelGGL((k_bs_partial_f64_grp<4, 1>), dim3((unsigned)nwork), dim3(kWG), 0, st,
+                                   (const double*)seg, work, pts, ptot, partial);
+            return hipGetLastError();
+        }
         if (precision != 33) return hipErrorInvalidValue;
         if (ppl == -2)
             hipLaunchKernelGGL(k_bs_partial_f32k_row, dim3((unsigned)nwork), dim3(kWG), 0, st,

@@ -84,8 +84,8 @@ int cctamd_pack_fp32(const double *seg64, int64_t nseg, float *seg32, double *ti
  * Host-only planner: splits (design, point block, tile chunk) work.
  *   design_ntile[d], design_npt[d], design_tile_off[d], design_pt_off[d] (host).
  *   ppl: points per lane (1, 2 or 4): a point block holds 64*ppl points; or -2 / -4:
- *        blocks of 32 / 16 points, each wave split into 2 / 4 lane groups walking
- *        different tiles (precision 33 only).
+ *        blocks of 32 / 16 points, each wave split into lane groups walking
+ *        different tiles (precisions 33 and 64; not 32).
  * With work == NULL returns the number of work items only. Otherwise fills
  * work[nwork][8] = {design, pt_start, pt_count, tile_start, tile_count, chunk, nchunk, block},
  * design_wbase[d] (first item of design d) and design_nchunk[d].

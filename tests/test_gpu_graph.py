@@ -16,7 +16,7 @@ if not gpu_available():
     pytest.skip("no GPU visible", allow_module_level=True)
 
 
-@pytest.mark.parametrize("precision,ppl", [(64, 2), (33, 2), (33, -2)])
+@pytest.mark.parametrize("precision,ppl", [(64, 2), (64, -2), (33, 2), (33, -2)])
 def test_graph_replays_match_eager(precision, ppl):
     wl = W.population(n_designs=3, n_particles=16, seed=11)
     bls = [W.build_beamline(d) for d in wl.designs]

@@ -135,9 +135,8 @@ def test_field_fp32_kc_next_to_windings(ppl):
     assert err.max() < 2e-5, err.max()
 
 
-@pytest.mark.parametrize("ppl", [-2, -4])
-def test_field_fp32_small_blocks_batched(ppl):
-    """Split-wave kernel (32- or 16-point blocks) on two designs with uneven point and segment counts."""
+def _two_uneven_designs():
+    """Two population designs with 45 and 13 in-bore points, and the oracle's field at them."""
     from cctpy_amd import workloads as W
     from oracle import workload_builder as WB
     wl = W.population(n_designs=2, n_particles=8, seed=3)
@@ -156,14 +155,32 @@ def test_field_fp32_small_blocks_batched(ppl):
         npt.append(n)
     model = engine.model_for([device_parts(W.build_beamline(d)).spec() for d in wl.designs])
     assert model.design_nseg[0] != model.design_nseg[1]
-    got = model.field(np.concatenate(pts_all), npt, precision=33, ppl=ppl)
-    want = np.concatenate(want_all)
+    return model, np.concatenate(pts_all), npt, np.concatenate(want_all)
+
+
+@pytest.mark.parametrize("ppl", [-2, -4])
+def test_field_fp32_small_blocks_batched(ppl):
+    """Split-wave kernel (32- or 16-point blocks) on two designs with uneven point and segment counts."""
+    model, pts, npt, want = _two_uneven_designs()
+    got = model.field(pts, npt, precision=33, ppl=ppl)
     err = np.linalg.norm(got - want, axis=1) / np.linalg.norm(want, axis=1)
     assert err.max() < 1e-6, err.max()
-    ref2 = model.field(np.concatenate(pts_all), npt, precision=33, ppl=2)
+    ref2 = model.field(pts, npt, precision=33, ppl=2)
     assert np.max(np.linalg.norm(got - ref2, axis=1) / np.linalg.norm(want, axis=1)) < 1e-6
     with pytest.raises(Exception):
-        model.field(np.concatenate(pts_all), npt, precision=64, ppl=ppl)
+        model.field(pts, npt, precision=32, ppl=ppl)
+
+
+@pytest.mark.parametrize("ppl", [-2, -4])
+def test_field_fp64_small_blocks_batched(ppl):
+    """FP64 split-wave kernel: same pair terms as the 64-point kernel, summed in another tile
+    grouping, so it agrees with the oracle to 1e-12 and with the 64-point kernel to rounding."""
+    model, pts, npt, want = _two_uneven_designs()
+    got = model.field(pts, npt, precision=64, ppl=ppl)
+    err = np.linalg.norm(got - want, axis=1) / np.linalg.norm(want, axis=1)
+    assert err.max() < 1e-12, err.max()
+    ref2 = model.field(pts, npt, precision=64, ppl=2)
+    assert np.max(np.linalg.norm(got - ref2, axis=1) / np.linalg.norm(want, axis=1)) < 1e-13
 
 
 def test_qs_fields_bitwise(golden):

@@ -90,8 +90,8 @@ def test_plan_rejects_bad_arguments():
     assert N.lib.cctamd_plan(0, p(z), p(z), p(z), p(z), 2, 10, 4, None, None, None) == -22
     neg = np.array([-1], np.int64)
     assert N.lib.cctamd_plan(1, p(neg), p(z), p(z), p(z), 2, 10, 4, None, None, None) == -22
-    # split-wave blocks exist only for the K x q - C FP32 kernel
-    assert N.lib.cctamd_bs_partial(64, -2, p(z), p(z), p(z), 1, p(z), 1, p(z), None) == -22
+    # split-wave blocks exist for precisions 33 and 64, not for the re-centred FP32 kernel (32)
+    assert N.lib.cctamd_bs_partial(32, -2, p(z), p(z), p(z), 1, p(z), 1, p(z), None) == -22
 
 
 def test_adjoint_requires_fp32_tile_origins():
