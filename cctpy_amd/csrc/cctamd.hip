@@ -321,27 +321,49 @@ struct KCRow {
     float4 a, b, c;  // kl.xyz, C.x | C.yz, -2m.xy | -2m.z, |m|^2, 0, 0
 };
 
+// |q - m|^2 of segment S (lane S of the row) for both points of each lane, as
+// ((|q|^2 + |m|^2) + qx (-2m.x)) + qy (-2m.y) + qz (-2m.z) like k_bs_partial_f32k,
+// in one assembly block: the compiler pads every dependence between two separate
+// inline-assembly statements with a wait state, which these plain accumulator
+// dependences do not need (their broadcast sources are loaded, not computed).
+template <int S>
+__device__ __forceinline__ void r2_bcast(const KCRow& g, const float (&qx)[2], const float (&qy)[2],
+                                         const float (&qz)[2], const float (&qq)[2], float& r0, float& r1) {
+    asm volatile(
+        "v_add_f32_dpp %0, %2, %3 row_newbcast:%14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_add_f32_dpp %1, %2, %4 row_newbcast:%14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %0, %5, %6 row_newbcast:%14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %1, %5, %7 row_newbcast:%14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %0, %8, %9 row_newbcast:%14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %1, %8, %10 row_newbcast:%14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %0, %11, %12 row_newbcast:%14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %1, %11, %13 row_newbcast:%14 row_mask:0xf bank_mask:0xf"
+        : "=&v"(r0), "=&v"(r1)
+        : "v"(g.c.y), "v"(qq[0]), "v"(qq[1]), "v"(g.b.z), "v"(qx[0]), "v"(qx[1]), "v"(g.b.w), "v"(qy[0]),
+          "v"(qy[1]), "v"(g.c.x), "v"(qz[0]), "v"(qz[1]), "i"(S));
+}
+
 template <int S>
 __device__ __forceinline__ void kc_row_steps(const KCRow& g, const float (&qx)[2], const float (&qy)[2],
                                              const float (&qz)[2], const float (&qq)[2], float (&kx)[2],
                                              float (&ky)[2], float (&kz)[2], float (&cx)[2], float (&cy)[2],
                                              float (&cz)[2]) {
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        // ((|q|^2 + |m|^2) + qx (-2m.x)) + qy (-2m.y) + qz (-2m.z), as in k_bs_partial_f32k
-        float r2 = add_bcast<S>(g.c.y, qq[j]);
-        fmac_bcast<S>(r2, g.b.z, qx[j]);
-        fmac_bcast<S>(r2, g.b.w, qy[j]);
-        fmac_bcast<S>(r2, g.c.x, qz[j]);
-        const float inv = __builtin_amdgcn_rsqf(r2);
-        const float w3 = inv * inv * inv;
-        fmac_bcast<S>(kx[j], g.a.x, w3);
-        fmac_bcast<S>(ky[j], g.a.y, w3);
-        fmac_bcast<S>(kz[j], g.a.z, w3);
-        fmac_bcast<S>(cx[j], g.a.w, w3);
-        fmac_bcast<S>(cy[j], g.b.x, w3);
-        fmac_bcast<S>(cz[j], g.b.y, w3);
-    }
+    float r0, r1;
+    r2_bcast<S>(g, qx, qy, qz, qq, r0, r1);
+    const float i0 = __builtin_amdgcn_rsqf(r0), i1 = __builtin_amdgcn_rsqf(r1);
+    const float w0 = i0 * i0 * i0, w1 = i1 * i1 * i1;
+    fmac_bcast<S>(kx[0], g.a.x, w0);
+    fmac_bcast<S>(kx[1], g.a.x, w1);
+    fmac_bcast<S>(ky[0], g.a.y, w0);
+    fmac_bcast<S>(ky[1], g.a.y, w1);
+    fmac_bcast<S>(kz[0], g.a.z, w0);
+    fmac_bcast<S>(kz[1], g.a.z, w1);
+    fmac_bcast<S>(cx[0], g.a.w, w0);
+    fmac_bcast<S>(cx[1], g.a.w, w1);
+    fmac_bcast<S>(cy[0], g.b.x, w0);
+    fmac_bcast<S>(cy[1], g.b.x, w1);
+    fmac_bcast<S>(cz[0], g.b.y, w0);
+    fmac_bcast<S>(cz[1], g.b.y, w1);
     if constexpr (S + 1 < 16) kc_row_steps<S + 1>(g, qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
 }
 

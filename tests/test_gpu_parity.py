@@ -253,7 +253,7 @@ def test_batched_designs_match_golden(golden):
     assert p1.distance == 0.0  # inputs untouched
 
 
-@pytest.mark.parametrize("ppl", [1, 2, 4])
+@pytest.mark.parametrize("ppl", [1, 2, 4, -2, -4])
 def test_tracking_beam_fp64_vs_c_oracle(ppl):
     """70 protons (not a multiple of 64), 1 mm steps through a 720-segment/winding AG-CCT section slice."""
     bl = HUST_SC_GANTRY(part_per_winding=720).create_second_bending_part_beamline()
