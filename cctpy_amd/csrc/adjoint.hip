@@ -619,7 +619,7 @@ __global__ __launch_bounds__(kWG) void k_adj_stage(int ppl, const double* __rest
     const PointSlot sl = point_slot(g, ppl, partial, design_wbase, design_nchunk, design_pt_off, nd);
     if (!active) return;
     double Pb[3];
-    chunk_sum(sl, Pb);
+    chunk_sum(sl, Pb, adj);
     if ((threadIdx.x & (kWave - 1)) != 0) return;
     qs_adjoint(sl.design, qs, design_qs_off, Pn, Bn, Pb, qgrad, ptot, g);
     for (int i = 0; i < 3; ++i) ps[i] = ps[i] + Pb[i];

@@ -756,7 +756,7 @@ __global__ __launch_bounds__(kWG) void k_field_reduce(int ppl, const double* __r
     const double p[3] = {pts[g], pts[ptot + g], pts[2 * ptot + g]};
     const PointSlot sl = point_slot(g, ppl, partial, design_wbase, design_nchunk, design_pt_off, nd);
     double b[3];
-    chunk_sum(sl, b);
+    chunk_sum(sl, b, pts);
     if ((threadIdx.x & (kWave - 1)) != 0) return;
     analytic_fields(sl.design, qs, design_qs_off, bconst, p, b);
     B[g] = b[0];
@@ -808,6 +808,7 @@ __global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __rest
     // finished-particle test comes after the chunk sum, so that the compiler does not wait for
     // nsteps before issuing the rest (finished particles keep their state).
     const PointSlot sl = load_slot(scratch, ptot, g, partial);
+    const AnalyticSet an = analytic_set(sl.design, design_qs_off, bconst);
     const int64_t ns = nsteps[g];
     const double k = kq[g];
     const double h = dt[g];
@@ -824,10 +825,10 @@ __global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __rest
         }
     }
     double b[3];
-    chunk_sum(sl, b);
+    chunk_sum(sl, b, scratch);
     if (step >= ns) return;
     if ((threadIdx.x & (kWave - 1)) != 0) return;
-    analytic_fields(sl.design, qs, design_qs_off, bconst, p, b);
+    analytic_fields(an, qs, p, b);
     if (tape) {  // stage point, velocity and field for the adjoint
         double* t = tape + S * 9 * ptot;
         for (int c = 0; c < 3; ++c) {
