@@ -76,6 +76,38 @@ def test_field_fp64_matches_golden_and_oracle(golden, ppl):
     assert err.max() < 1e-9, err.max()
 
 
+@pytest.mark.parametrize("target", [1, 64, 4000, 20000])
+def test_field_fp64_any_chunk_count(target):
+    """Plans from a few chunks per point block to more than 256 (several rounds of the
+    fixed-order chunk sum, whose lanes load four chunks at a time, the last round partly
+    empty): FP64 field at in-bore points of a 368,640-segment AG-CCT section vs the oracle."""
+    import torch
+    from cctpy_amd import workloads as W
+    from oracle import workload_builder as WB
+    wl = W.agcct_section(n_particles=8)
+    bl = W.build_beamline(wl.designs[0])
+    ref = WB.build_beamline(wl.designs[0])
+    rng = np.random.default_rng(13)
+    pts = []
+    for s in rng.uniform(0.3, bl.get_length() - 0.3, 100):
+        q, d = ref.traj.point_at(float(s)), ref.traj.direct_at(float(s))
+        dx, dz = rng.uniform(-0.03, 0.03), rng.uniform(-0.03, 0.03)
+        pts.append((q[0] - d[1] * dx, q[1] + d[0] * dx, dz))
+    pts = np.array(pts)
+    model = engine.model_for([device_parts(bl).spec()])
+    plan = model.plan([len(pts)], 1, target)
+    nc = int(plan.nchunk.max())
+    assert nc <= 8 if target == 1 else (nc > 256 if target >= 4000 else 8 < nc <= 256), nc
+    _, partial = model.workspace(plan)
+    p = torch.tensor(pts.T.copy(), dtype=torch.float64, device=engine.device())
+    out = torch.empty_like(p)
+    model.field_tensors(plan, 64, p, partial, out)
+    got = out.cpu().numpy().T
+    want = C.field(O.beamline_arrays(ref.magnets), pts)
+    err = np.linalg.norm(got - want, axis=1) / np.linalg.norm(want, axis=1)
+    assert err.max() < 1e-12, (nc, err.max())
+
+
 def test_field_fp32_recentred_accuracy():
     ours, ref = _c05_pair()
     model = engine.model_for([device_parts(ours).spec()])
