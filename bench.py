@@ -222,10 +222,12 @@ class Setup:
         + objective + adjoint pass, and the split.  With ``tune`` the block shape is the one
         with the fastest short forward + adjoint pass; otherwise ``shape``'s."""
         import torch
+        from cctpy_amd import engine
         from cctpy_amd.adjoint import AdjointTracker
         from cctpy_amd.beam import BeamGroups
         from cctpy_amd.particles import ParticleFactory
-        tr = AdjointTracker(self.bls, precision=FP32, ckpt_every=ckpt_every, ppl=shape["ppl"])
+        ipc = shape["target_items"] // engine.cu_count() if shape.get("target_items") else None
+        tr = AdjointTracker(self.bls, precision=FP32, ckpt_every=ckpt_every, ppl=shape["ppl"], items_per_cu=ipc)
         if tune:  # forward + backward decide the block shape, not the forward kernel alone
             tr.tune(self.npt, self.s0, self.kq, self.dt, self.ns_t)
         tr.prepare(self.npt, self.kq, self.dt, self.ns_t, self.max_steps)
@@ -250,6 +252,7 @@ class Setup:
         J = BeamGroups.objective(out["stats"])
         return {"ms_per_pass": ms, "forward_ms": fwd, "adjoint_ms": ms - fwd,
                 "particle_steps_per_s": self.particle_steps / (ms / 1e3), "ppl": tr.plan.ppl,
+                "items_per_cu": tr.items_per_cu,
                 "checkpoint_every": ckpt_every, "objective_mean": float(J.mean()),
                 "grad_finite": bool(torch.isfinite(tr.tgrad).all().item())}
 

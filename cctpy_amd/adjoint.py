@@ -27,10 +27,12 @@ from .magnets import Magnet, device_parts
 class AdjointTracker:
     """Forward tracking with checkpoints and its adjoint for one set of designs."""
 
-    def __init__(self, magnets: Sequence[Magnet], precision: int = 33, ckpt_every: int = 50, ppl: int = 2):
+    def __init__(self, magnets: Sequence[Magnet], precision: int = 33, ckpt_every: int = 50, ppl: int = 2,
+                 items_per_cu: Optional[int] = None):
         """``ckpt_every`` K > 0: checkpoints every K steps and recompute in the reverse pass;
         K = 0: the forward keeps the tape of every step (288 B per particle-step of HBM) and the
-        reverse pass recomputes nothing."""
+        reverse pass recomputes nothing.  ``ppl`` is the points-per-lane code of the work plan,
+        ``items_per_cu`` its work items per CU (None: the engine's default)."""
         if ckpt_every < 0:
             raise ValueError("ckpt_every must be >= 0")
         specs = []
@@ -44,6 +46,7 @@ class AdjointTracker:
         self.precision = precision
         self.ckpt_every = int(ckpt_every)
         self.ppl = ppl
+        self.items_per_cu = items_per_cu
 
     def prepare(self, design_npt: Sequence[int], kq: torch.Tensor, dt: torch.Tensor, nsteps: torch.Tensor,
                 max_steps: int) -> None:
@@ -51,7 +54,8 @@ class AdjointTracker:
         device tensors (P,) that stay bound to this tracker."""
         dev = engine.device()
         m = self.model
-        self.plan = plan = m.plan(design_npt, self.ppl)
+        target = self.items_per_cu * engine.cu_count() if self.items_per_cu else None
+        self.plan = plan = m.plan(design_npt, self.ppl, target)
         self.P = P = plan.npt
         self.kq, self.dt, self.ns = kq, dt, nsteps
         self.max_steps = int(max_steps)
@@ -75,15 +79,17 @@ class AdjointTracker:
         self.bs_wts = torch.zeros((3, P), dtype=torch.float64, device=dev)
 
     def tune(self, design_npt: Sequence[int], state: torch.Tensor, kq: torch.Tensor, dt: torch.Tensor,
-             nsteps: torch.Tensor, candidates: Sequence[int] = (1, 2, -2), probe_steps: int = 8) -> int:
-        """Sets (and returns) the points-per-lane code whose forward + adjoint pass over the first
-        ``probe_steps`` steps is fastest.  The backward kernel dominates the adjoint and prefers
-        other block shapes than the forward alone (2 points per lane against 1 on a 256-proton
-        section), so the shape is chosen on both; call prepare() afterwards."""
+             nsteps: torch.Tensor, candidates: Sequence[int] = (1, 2, -2), probe_steps: int = 8,
+             items_per_cu: Sequence[int] = (4, 8, 16)) -> int:
+        """Sets the points-per-lane code and the work items per CU whose forward + adjoint pass
+        over the first ``probe_steps`` steps is fastest, and returns the code.  The backward
+        kernel dominates the adjoint and prefers other block shapes than the forward alone (2
+        points per lane against 1 on a 256-proton section), so the shape is chosen on both; call
+        prepare() afterwards."""
         ns = torch.clamp(nsteps, max=probe_steps)
-        best, best_ms = self.ppl, math.inf
-        for ppl in candidates:
-            self.ppl = ppl
+        best, best_ms = (self.ppl, self.items_per_cu), math.inf
+        for ppl, ipc in [(a, b) for a in candidates for b in items_per_cu]:
+            self.ppl, self.items_per_cu = ppl, ipc
             self.prepare(design_npt, kq, dt, ns, probe_steps)
             seed = torch.zeros((6, self.P), dtype=torch.float64, device=state.device)
             seed[:3] = 1.0
@@ -98,9 +104,9 @@ class AdjointTracker:
                 torch.cuda.synchronize()
                 times.append(e0.elapsed_time(e1))
             if times[-1] < best_ms:
-                best, best_ms = ppl, times[-1]
-        self.ppl = best
-        return best
+                best, best_ms = (ppl, ipc), times[-1]
+        self.ppl, self.items_per_cu = best
+        return self.ppl
 
     def forward_tensors(self, state: torch.Tensor) -> None:
         """In-place tracking of device state (6, P), keeping checkpoints every K steps (or, with

@@ -221,8 +221,9 @@ def test_adjoint_full_tape_matches_checkpoints():
 
 
 def test_adjoint_tune_picks_a_candidate_and_keeps_results():
-    """tune() times short forward + adjoint passes, sets one of the candidate block shapes,
-    and a tracker prepared after it gives the bits of a fresh tracker with that shape."""
+    """tune() times short forward + adjoint passes, sets one of the candidate block shapes and
+    work-item counts, and a tracker prepared after it gives the bits of a fresh tracker with
+    that shape."""
     from cctpy_amd import engine
     wl = W.population(n_designs=2, n_particles=16, seed=6)
     bls = [W.build_beamline(d) for d in wl.designs]
@@ -242,10 +243,10 @@ def test_adjoint_tune_picks_a_candidate_and_keeps_results():
     kq_t, dt_t, ns_t = (torch.from_numpy(a).to(dev) for a in (kq, dt, ns))
     w = np.random.default_rng(5).normal(size=(len(ps), 6)) * np.array([1.0, 1.0, 1.0, 1e-8, 1e-8, 1e-8])
     tr = adjoint.AdjointTracker(bls, precision=33, ckpt_every=16, ppl=2)
-    best = tr.tune(npt, s0, kq_t, dt_t, ns_t, candidates=(1, 2, -2), probe_steps=6)
-    assert best in (1, 2, -2) and tr.ppl == best
+    best = tr.tune(npt, s0, kq_t, dt_t, ns_t, candidates=(1, 2, -2), probe_steps=6, items_per_cu=(4, 16))
+    assert best in (1, 2, -2) and tr.ppl == best and tr.items_per_cu in (4, 16)
     out = []
-    for t in (tr, adjoint.AdjointTracker(bls, precision=33, ckpt_every=16, ppl=best)):
+    for t in (tr, adjoint.AdjointTracker(bls, precision=33, ckpt_every=16, ppl=best, items_per_cu=tr.items_per_cu)):
         final = t.forward(state, kq, dt, ns, npt)
         a0, cg, qg = t.backward(w)
         out.append((final, a0, cg))
