@@ -676,9 +676,10 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f64_grp(const double* __rest
     constexpr int LG = kWave / G;         // lanes per group
     constexpr int PB = LG * PL;           // points per block
     constexpr int NS = kWavesPerWG * G;   // tile streams per workgroup
-    constexpr int ROW = kTile * 4;        // float4 (one double2 each) per tile
-    constexpr int HALF = ROW / PARTS;     // float4 per staged part of a tile
-    __shared__ float4 lds[NS][HALF + 1];
+    constexpr int ROW = kTile * 4;        // float4 (one double2 each) per tile in seg64
+    constexpr int SEGS = kTile / PARTS;   // segments per staged part of a tile
+    // LDS rows drop seg64's two pad doubles: (kl.x, kl.y), (kl.z, m.x), (m.y, m.z)
+    __shared__ float4 lds[NS][3 * SEGS + 1];
     __shared__ double red[kWavesPerWG][3][PB];
     const Work w = work[blockIdx.x];
     const int lane = threadIdx.x & (kWave - 1);
@@ -698,39 +699,36 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f64_grp(const double* __rest
     }
     const int tend = w.tile_start + w.tile_count;
     const float4* src = reinterpret_cast<const float4*>(seg64);
-    constexpr int N = NS * HALF;
-    constexpr int IT = N / kWG;
-    static_assert(N % kWG == 0, "whole staging rounds");
+    static_assert(NS * SEGS == kWG, "one segment per thread and staging round");
+    const int st = (int)threadIdx.x / SEGS, sk = (int)threadIdx.x % SEGS;
     for (int t0 = w.tile_start; t0 < tend; t0 += NS) {
 #pragma unroll 1
         for (int h = 0; h < PARTS; ++h) {
-            float4 v[IT];
-#pragma unroll
-            for (int b = 0; b < IT; ++b) {  // every load of the part in flight at once
-                const int i = (int)threadIdx.x + b * kWG;
-                const int st = i / HALF, e = i % HALF;
-                v[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // rows of absent tiles: not read
-                if (t0 + st < tend) v[b] = src[(int64_t)(t0 + st) * ROW + h * HALF + e];
+            float4 v0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v1 = v0, v2 = v0, v3 = v0;  // absent tiles: not read
+            if (t0 + st < tend) {
+                const float4* g = src + (int64_t)(t0 + st) * ROW + (h * SEGS + sk) * 4;
+                v0 = g[0];  // kl.x, kl.y
+                v1 = g[1];  // kl.z, pad
+                v2 = g[2];  // m.x, m.y
+                v3 = g[3];  // m.z, pad
             }
-#pragma unroll
-            for (int b = 0; b < IT; ++b) {
-                const int i = (int)threadIdx.x + b * kWG;
-                lds[i / HALF][i % HALF] = v[b];
-            }
+            lds[st][3 * sk] = v0;
+            lds[st][3 * sk + 1] = make_float4(v1.x, v1.y, v2.x, v2.y);
+            lds[st][3 * sk + 2] = make_float4(v2.z, v2.w, v3.x, v3.y);
             __syncthreads();
             if (t0 + stream < tend) {  // uniform within a lane group
                 const double2* r = reinterpret_cast<const double2*>(lds[stream]);
 #pragma unroll 4
-                for (int k = 0; k < kTile / PARTS; ++k) {
-                    const double2 k01 = r[4 * k], k2_ = r[4 * k + 1], m01 = r[4 * k + 2], m2_ = r[4 * k + 3];
+                for (int k = 0; k < SEGS; ++k) {
+                    const double2 k01 = r[3 * k], k2m0 = r[3 * k + 1], m12 = r[3 * k + 2];
 #pragma unroll
                     for (int j = 0; j < PL; ++j) {
-                        const double rx = px[j] - m01.x, ry = py[j] - m01.y, rz = pz[j] - m2_.x;
+                        const double rx = px[j] - k2m0.y, ry = py[j] - m12.x, rz = pz[j] - m12.y;
                         const double r2 = rx * rx + ry * ry + rz * rz;
                         const double inv = rsqrt_f64(r2);
                         const double inv3 = inv * inv * inv;
-                        ax[j] += (k01.y * rz - k2_.x * ry) * inv3;
-                        ay[j] += (k2_.x * rx - k01.x * rz) * inv3;
+                        ax[j] += (k01.y * rz - k2m0.x * ry) * inv3;
+                        ay[j] += (k2m0.x * rx - k01.x * rz) * inv3;
                         az[j] += (k01.x * ry - k01.y * rx) * inv3;
                     }
                 }
