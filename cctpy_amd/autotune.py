@@ -49,13 +49,22 @@ def _store(key: str, val: dict) -> None:
 
 # Bumped whenever the kernels or the candidate set change, so that cached choices of an
 # older library are not reused.
-KERNEL_SET = 5
+KERNEL_SET = 6
+TIE = 0.015
 
 
 def _key(model: "engine.DeviceModel", npt: Sequence[int], precision: int) -> str:
     props = torch.cuda.get_device_properties(engine.device())
     return f"v{KERNEL_SET}|{props.name}|cu{props.multi_processor_count}|fp{precision}|" \
            f"seg{list(map(int, model.design_nseg))}|pt{list(map(int, npt))}"
+
+
+def pick(cost: Dict[tuple, float], tie: float = TIE) -> tuple:
+    """((ppl, items per CU), ms) from measured costs.  Shapes within ``tie`` of the fastest
+    are a tie (timing noise is about 1 %): the one with the most work items per CU wins, its
+    finer chunks balancing better as the particles move along the track."""
+    fastest = min(cost.values())
+    return max(((k, v) for k, v in cost.items() if v <= fastest * (1.0 + tie)), key=lambda kv: (kv[0][1], -kv[1]))
 
 
 def tune(model: "engine.DeviceModel", npt: Sequence[int], precision: int, points,
@@ -94,7 +103,7 @@ def tune(model: "engine.DeviceModel", npt: Sequence[int], precision: int, points
                 ms = e0.elapsed_time(e1) / (launches * len(sets))
                 k = (ppl, per_cu)
                 cost[k] = min(cost.get(k, ms), ms)
-    (ppl, per_cu), ms = min(cost.items(), key=lambda kv: kv[1])
+    (ppl, per_cu), ms = pick(cost)
     best = {"ppl": ppl, "target_items": per_cu * cus, "ms": ms}
     _MEM[key] = best
     if use_cache:

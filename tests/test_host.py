@@ -191,3 +191,13 @@ def test_integration_binding_matches_the_loader():
         m = re.search(r"lib\." + name + r"\.argtypes = (\[[^\]]*\])", native)
         assert m, name
         assert re.sub(r"\s", "", args) == re.sub(r"\s", "", m.group(1)), name
+
+
+def test_autotune_pick_breaks_near_ties_toward_more_items():
+    from cctpy_amd.autotune import pick
+    cost = {(1, 4): 0.0797, (1, 8): 0.0804, (2, 8): 0.0830, (1, 16): 0.0850}
+    assert pick(cost) == ((1, 8), 0.0804)            # 0.9 % slower: a tie, more items win
+    assert pick(cost, tie=0.005) == ((1, 4), 0.0797)  # outside the tie band the fastest wins
+    assert pick({(2, 4): 1.0}) == ((2, 4), 1.0)
+    # equal items per CU: the faster shape
+    assert pick({(1, 8): 0.0805, (2, 8): 0.0800}) == ((2, 8), 0.0800)
