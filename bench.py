@@ -347,6 +347,8 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true", help="skip the population and adjoint extras")
     ap.add_argument("--cpu-sample", default="64x24", help="CPU baseline sample: particles x RK4 steps")
     ap.add_argument("--roofline-launches", type=int, default=20)
+    ap.add_argument("--roofline-precision", type=int, choices=(FP32, 64), default=FP32,
+                    help="with --roofline-only: launch the FP32 (33) or the plain FP64 (64) kernel")
     ap.add_argument("--roofline-only", action="store_true", help="only launch the Biot-Savart kernel (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0, help="plain copy of this many MiB (counter calibration)")
     ap.add_argument("--graph", action="store_true", help="replay each tracking pass as one captured HIP graph")
@@ -380,14 +382,14 @@ def main() -> int:
     tuned = {FP32: S.shape(FP32, args.ppl, args.items_per_cu)}
     log(f"[bench] fp32 shape: {tuned[FP32]}")
 
-    def bs_launch_timing(plan, pts, partial, launches: int) -> float:
-        """Average duration (ms) of the FP32 Biot-Savart kernel, HIP events on its stream."""
-        model.bs_partial_tensors(plan, FP32, pts, partial)
+    def bs_launch_timing(plan, pts, partial, launches: int, prec: int = FP32) -> float:
+        """Average duration (ms) of the Biot-Savart kernel, HIP events on its stream."""
+        model.bs_partial_tensors(plan, prec, pts, partial)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(launches):
-            model.bs_partial_tensors(plan, FP32, pts, partial)
+            model.bs_partial_tensors(plan, prec, pts, partial)
         e1.record()
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / launches
@@ -395,8 +397,9 @@ def main() -> int:
     if args.roofline_only:
         plan = model.plan(S.npt, tuned[FP32]["ppl"], tuned[FP32]["target_items"])
         _, partial = model.workspace(plan)
-        avg = bs_launch_timing(plan, S.s0[:3].contiguous(), partial, args.roofline_launches)
-        print(json.dumps({"workload": wl.name, "kernel": kernel_name(plan.ppl), "work_items": plan.nwork,
+        avg = bs_launch_timing(plan, S.s0[:3].contiguous(), partial, args.roofline_launches, args.roofline_precision)
+        kname = kernel_name(plan.ppl) if args.roofline_precision == FP32 else f"k_bs_partial_f64<{plan.ppl}, {BS_WAVES}>"
+        print(json.dumps({"workload": wl.name, "kernel": kname, "work_items": plan.nwork,
                           "launches": args.roofline_launches + 1, "avg_launch_ms": avg}), flush=True)
         return 0
 
