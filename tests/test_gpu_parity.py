@@ -310,6 +310,26 @@ def test_tracking_beam_fp64_vs_c_oracle(ppl):
     assert np.max(np.abs(out[:, 3:] - want[:, 3:])) / 1.8e8 < 1e-12
 
 
+@pytest.mark.parametrize("ppl", [-2, -4])
+def test_tracking_beam_fp32_small_blocks_vs_fp64(ppl):
+    """FP32 tracking with the split-wave kernels (32- and 16-point blocks, 70 protons: ragged last
+    block) against the FP64 path over 30 steps of 1 mm through an AG-CCT section slice."""
+    from cctpy_amd.particles import PhaseSpaceParticle
+    bl = HUST_SC_GANTRY(part_per_winding=720).create_second_bending_part_beamline()
+    rng = np.random.default_rng(17)
+    ip = ParticleFactory.create_proton_along(bl, 2.0, 250)
+    cs = ip.get_natural_coordinate_system()
+    ps = []
+    for _ in range(70):
+        x, xp, y, yp = rng.normal(0, [1.75e-3, 2e-3, 1.75e-3, 2e-3])
+        ps.append(ParticleFactory.create_from_phase_space_particle(ip, cs, PhaseSpaceParticle(x, xp, y, yp, 0,
+                                                                                           rng.uniform(-0.02, 0.02))))
+    out64, _, _, _ = track_particles(ps, [bl], [0] * len(ps), 0.03, 1e-3, precision=64, ppl=2)
+    out32, _, _, _ = track_particles(ps, [bl], [0] * len(ps), 0.03, 1e-3, precision=33, ppl=ppl)
+    assert np.max(np.abs(out32[:, :3] - out64[:, :3])) < 1e-9
+    assert np.max(np.abs(out32[:, 3:] - out64[:, 3:])) / 1.8e8 < 1e-9
+
+
 def test_edge_cases():
     bl = HUST_SC_GANTRY(**SECTION1_DEMO).create_first_bending_part_beamline()
     assert ParticleRunner.run_only([], bl, 1.0) == []
