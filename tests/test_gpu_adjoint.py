@@ -82,10 +82,12 @@ def test_adjoint_beam_spot_objective_torch():
     assert np.max(np.abs(g - fd)) / np.abs(fd).max() < 1e-4, (g, fd)
 
 
-def test_adjoint_population_current_gradients_vs_fd():
+@pytest.mark.parametrize("ppl", [1, -2])
+def test_adjoint_population_current_gradients_vs_fd(ppl):
     """Two designs of different arc lengths (particles finish at different steps) whose
     windings do not fill their last tile: every gradient is finite and dJ/d(current) of each
-    CCT matches central differences of the FP64 forward path."""
+    CCT matches central differences of the FP64 forward path (FP64 forward with 64- or
+    32-point blocks)."""
     import copy
     from cctpy_amd import engine
     from cctpy_amd.magnets import device_parts
@@ -105,7 +107,7 @@ def test_adjoint_population_current_gradients_vs_fd():
     ns = np.array([s for s, _ in sd], np.int64)
     dt = np.array([d for _, d in sd])
     assert len(set(ns.tolist())) > 1
-    tr = adjoint.AdjointTracker(bls, precision=64, ckpt_every=16, ppl=1)
+    tr = adjoint.AdjointTracker(bls, precision=64, ckpt_every=16, ppl=ppl)
     final = tr.forward(state, kq, dt, ns, npt)
     rng = np.random.default_rng(8)
     w = rng.normal(size=final.shape) * np.array([1.0, 1.0, 1.0, 1e-8, 1e-8, 1e-8])
