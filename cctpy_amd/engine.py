@@ -56,6 +56,19 @@ def points_per_block(ppl: int) -> int:
     return 64 * ppl if ppl > 0 else 64 // (-ppl)
 
 
+def block_shape_for(design_npt: Sequence[int], precision: int) -> int:
+    """Points-per-lane code whose blocks waste the fewest lanes for these per-design particle
+    counts, for callers that do not tune: the smallest of 16, 32, 64 points per block that
+    holds the largest design's particles (16 and 32 only where split-wave kernels exist:
+    precisions 33 and 64), else 128 (2 points per lane)."""
+    m = int(numpy.max(design_npt)) if len(design_npt) else 0
+    if precision != 32 and m <= 16:
+        return -4
+    if precision != 32 and m <= 32:
+        return -2
+    return 1 if m <= 64 else 2
+
+
 def cu_count() -> int:
     return int(torch.cuda.get_device_properties(device()).multi_processor_count)
 

@@ -154,12 +154,14 @@ class RunningParticle:
 
 
 def track_particles(ps: Sequence[RunningParticle], magnets: Sequence[Magnet], design_of: Sequence[int],
-                    length: float, footstep: float, precision: int = 64, record: bool = False, ppl: int = 2):
+                    length: float, footstep: float, precision: int = 64, record: bool = False,
+                    ppl: Optional[int] = None):
     """Track particles (each through the design index design_of[i]) on the GPU.
 
     Returns (state (n, 6), record or None, per-particle step counts and dts),
     without mutating the inputs.  Designs with a magnet kind the device does
-    not know raise ValueError.
+    not know raise ValueError.  ``ppl`` (points-per-lane code) defaults to the
+    block shape that fits the largest design's particle count.
     """
     from . import engine
     specs = []
@@ -181,6 +183,8 @@ def track_particles(ps: Sequence[RunningParticle], magnets: Sequence[Magnet], de
         kq[row] = p.e / p.relativistic_mass
         ns[row], dt[row] = engine.rk4_steps(length, footstep, p.speed)
     model = engine.model_for(specs, fp32=(precision != 64))
+    if ppl is None:
+        ppl = engine.block_shape_for(npt, precision)
     out, rec = model.track(state, kq, dt, ns, npt, precision=precision, ppl=ppl, record=record)
     inv = numpy.empty(len(ps), numpy.int64)
     inv[numpy.asarray(order, numpy.int64)] = numpy.arange(len(ps))

@@ -201,3 +201,13 @@ def test_autotune_pick_breaks_near_ties_toward_more_items():
     assert pick({(2, 4): 1.0}) == ((2, 4), 1.0)
     # equal items per CU: the faster shape
     assert pick({(1, 8): 0.0805, (2, 8): 0.0800}) == ((2, 8), 0.0800)
+
+
+def test_block_shape_for_particle_counts():
+    from cctpy_amd.engine import block_shape_for, points_per_block
+    assert block_shape_for([8, 16], 64) == -4 and points_per_block(-4) == 16
+    assert block_shape_for([32, 5], 33) == -2 and points_per_block(-2) == 32
+    assert block_shape_for([33], 64) == 1
+    assert block_shape_for([16], 32) == 1      # no split-wave kernel for the re-centred FP32 field path
+    assert block_shape_for([65, 1], 64) == 2
+    assert block_shape_for([], 64) == -4       # nothing to track: any valid code
