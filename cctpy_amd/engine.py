@@ -253,9 +253,13 @@ class DeviceModel:
         raise ValueError("precision must be 32, 33 or 64")
 
     # ------------------------------------------------------------- field
-    def field(self, points: numpy.ndarray, design_npt: Sequence[int], precision: int = 64, ppl: int = 2) -> numpy.ndarray:
-        """B at points (P, 3) grouped by design (design_npt[d] consecutive rows each)."""
+    def field(self, points: numpy.ndarray, design_npt: Sequence[int], precision: int = 64,
+              ppl: Optional[int] = None) -> numpy.ndarray:
+        """B at points (P, 3) grouped by design (design_npt[d] consecutive rows each); ``ppl``
+        defaults to the block shape that fits the largest design's point count."""
         pts = numpy.asarray(points, numpy.float64).reshape(-1, 3)
+        if ppl is None:
+            ppl = block_shape_for(design_npt, precision)
         plan = self.plan(design_npt, ppl)
         if plan.npt != pts.shape[0]:
             raise ValueError("point count does not match design_npt")
