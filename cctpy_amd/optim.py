@@ -119,7 +119,7 @@ class GantryPopulationProblem:
         kq = torch.tensor([p.e / p.relativistic_mass for p in self.particles] * D, dtype=torch.float64, device=dev)
         dt = torch.tensor([d for _, d in sd], dtype=torch.float64, device=dev)
         ns = torch.tensor([s for s, _ in sd], dtype=torch.int64, device=dev)
-        plan = model.plan([n] * D, 2)
+        plan = model.plan([n] * D, engine.block_shape_for([n], self.precision))
         scratch, partial = model.workspace(plan)
         model.track_tensors(plan, self.precision, state, kq, dt, ns, int(ns.max()), scratch, partial)
         frame = self.ip_end.get_natural_coordinate_system()
