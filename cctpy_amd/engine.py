@@ -278,15 +278,18 @@ class DeviceModel:
 
     # ---------------------------------------------------------- tracking
     def track(self, state: numpy.ndarray, kq: numpy.ndarray, dt: numpy.ndarray, nsteps: numpy.ndarray,
-              design_npt: Sequence[int], precision: int = 64, ppl: int = 2,
+              design_npt: Sequence[int], precision: int = 64, ppl: Optional[int] = None,
               record: bool = False) -> Tuple[numpy.ndarray, Optional[numpy.ndarray]]:
         """RK4 tracking; state (P, 6) rows [x, y, z, vx, vy, vz] grouped by design.
 
         Returns the final state (P, 6) and, with ``record``, the state after
         every step as (max_steps+1, P, 6) (rows past a particle's own step
-        count repeat nothing meaningful and are masked by the caller).
+        count repeat nothing meaningful and are masked by the caller).  ``ppl``
+        defaults to the block shape that fits the largest design's particle count.
         """
         st = numpy.asarray(state, numpy.float64).reshape(-1, 6)
+        if ppl is None:
+            ppl = block_shape_for(design_npt, precision)
         plan = self.plan(design_npt, ppl)
         if plan.npt != st.shape[0]:
             raise ValueError("particle count does not match design_npt")
