@@ -44,20 +44,20 @@ def _load() -> ctypes.CDLL:
     lib.cctamd_pack_fp32.restype = INT
     lib.cctamd_plan.argtypes = [I64, P, P, P, P, INT, I64, I64, P, P, P]
     lib.cctamd_plan.restype = I64
-    lib.cctamd_bs_partial.argtypes = [INT, INT, P, P, P, I64, P, I64, P, P]
+    lib.cctamd_bs_partial.argtypes = [INT, INT, P, P, P, P, I64, P, I64, P, P]
     lib.cctamd_bs_partial.restype = INT
-    lib.cctamd_bs_field.argtypes = [INT, INT, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P]
+    lib.cctamd_bs_field.argtypes = [INT, INT, P, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P]
     lib.cctamd_bs_field.restype = INT
-    lib.cctamd_track_rk4.argtypes = [INT, INT, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P, I64, P, P, P,
-                                     I64, P]
+    lib.cctamd_track_rk4.argtypes = [INT, INT, P, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P, I64, P, P,
+                                     P, I64, P]
     lib.cctamd_track_rk4.restype = INT
     lib.cctamd_winding_tangents.argtypes = [P, P, P, I64, I64, P, P]
     lib.cctamd_winding_tangents.restype = INT
-    lib.cctamd_track_rk4_adjoint.argtypes = [INT, INT, P, P, P, P, P, P, I64, P, I64, P, P, P, I64, P, P, P,
+    lib.cctamd_track_rk4_adjoint.argtypes = [INT, INT, P, P, P, P, P, P, P, I64, P, I64, P, P, P, I64, P, P, P,
                                              P, I64, I64, P, P, P, I64, P, P, P, P, P, P, P, P, P, P, P]
     lib.cctamd_track_rk4_adjoint.restype = INT
-    lib.cctamd_track_rk4_taped.argtypes = [INT, INT, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P, I64, P,
-                                           P, P, P]
+    lib.cctamd_track_rk4_taped.argtypes = [INT, INT, P, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P, I64,
+                                           P, P, P, P]
     lib.cctamd_track_rk4_taped.restype = INT
     lib.cctamd_phase_space.argtypes = [P, I64, P, P, P, I64, P, P, I64, P, P, P]
     lib.cctamd_phase_space.restype = INT

@@ -23,6 +23,9 @@ from . import engine
 from .engine import _p, _stream
 from .magnets import Magnet, device_parts
 
+# tilt angles per CCT whose gradients the adjoint carries (kGradTilts in csrc/adjoint.hip)
+MAX_GRAD_TILTS = 4
+
 
 class AdjointTracker:
     """Forward tracking with checkpoints and its adjoint for one set of designs."""
@@ -84,8 +87,8 @@ class AdjointTracker:
         """Sets the points-per-lane code and the work items per CU whose forward + adjoint pass
         over the first ``probe_steps`` steps is fastest, and returns the code.  The backward
         kernel dominates the adjoint and prefers other block shapes than the forward alone (2
-        points per lane against 1 on a 256-proton section), so the shape is chosen on both; call
-        prepare() afterwards."""
+        points per lane against 1 on a 256-proton section), so the shape is chosen on both.  The
+        tracker is left prepared for ``nsteps`` with the chosen shape."""
         ns = torch.clamp(nsteps, max=probe_steps)
         best, best_ms = (self.ppl, self.items_per_cu), math.inf
         for ppl, ipc in [(a, b) for a in candidates for b in items_per_cu]:
@@ -106,6 +109,8 @@ class AdjointTracker:
             if times[-1] < best_ms:
                 best, best_ms = (ppl, ipc), times[-1]
         self.ppl, self.items_per_cu = best
+        # prepared for the caller's own steps again, not the probe's
+        self.prepare(design_npt, kq, dt, nsteps, int(nsteps.max().item()) if nsteps.numel() else 0)
         return self.ppl
 
     def forward_tensors(self, state: torch.Tensor) -> None:
@@ -116,9 +121,9 @@ class AdjointTracker:
                                      self.scratch, self.partial, record=self.ckpt, record_every=self.ckpt_every)
             return
         m, plan = self.model, self.plan
-        seg, org = m._seg(self.precision)
+        seg, org, near = m._seg(self.precision)
         N.check(N.lib.cctamd_track_rk4_taped(
-            self.precision, plan.ppl, _p(seg), _p(org), _p(plan.work), plan.nwork, _p(plan.wbase), _p(plan.nchunk),
+            self.precision, plan.ppl, _p(seg), _p(org), _p(near), _p(plan.work), plan.nwork, _p(plan.wbase), _p(plan.nchunk),
             _p(plan.design_pt_off), m.ndesign, _p(m.qs), _p(m.design_qs_off), _p(m.bconst), _p(state), self.P,
             _p(self.kq), _p(self.dt), _p(self.ns), self.max_steps, _p(self.scratch), _p(self.partial), _p(self.tape),
             _stream()), "cctamd_track_rk4_taped")
@@ -127,13 +132,13 @@ class AdjointTracker:
         """In-place reverse pass: adj (6, P) dJ/d(final state) -> dJ/d(initial state).
         Returns the device accumulators (tgrad (nblock, ntile, 5), qgrad (2Q, P))."""
         m, plan = self.model, self.plan
-        seg, _ = m._seg(self.precision)
+        seg, _, near = m._seg(self.precision)
         org = m.tile_org  # FP32 tile origins: the backward kernel needs them at every precision
         tan, tile_cct, coef = m.adjoint_data()
         self.tgrad.zero_()
         self.qgrad.zero_()
         N.check(N.lib.cctamd_track_rk4_adjoint(
-            self.precision, plan.ppl, _p(seg), _p(org), _p(m.seg32), _p(tan), _p(tile_cct), _p(coef), self.ntile,
+            self.precision, plan.ppl, _p(seg), _p(org), _p(near), _p(m.seg32), _p(tan), _p(tile_cct), _p(coef), self.ntile,
             _p(plan.work), plan.nwork, _p(plan.wbase), _p(plan.nchunk), _p(plan.design_pt_off), m.ndesign,
             _p(m.qs), _p(m.design_qs_off), _p(m.bconst), _p(self.ckpt), self.ckpt_every, self.P, _p(self.kq),
             _p(self.dt), _p(self.ns), self.max_steps, _p(adj), _p(self.tgrad), _p(self.qgrad), _p(self.state_ws),
@@ -212,52 +217,79 @@ class DesignProblem:
         return self.tracker.forward(state, kq, dt, ns, [len(state)])
 
     def _magnet_params(self, theta: numpy.ndarray):
-        """Per CCT (current, tilt angles, sinh eta) and per QS (g, s) of the design at theta."""
-        from .cct import CCT
-        from .magnets import QS
-        bl = self.builder(theta)
-        ccts, qss = [], []
-        for m in bl.magnets:
-            if isinstance(m, CCT):
-                ccts.append(m)
-            elif isinstance(m, QS):
-                qss.append(m)
-        return ccts, qss
+        """The CCTs and QS of the design at theta, flattened as the device holds them (nested
+        CombinedMagnets included, append order)."""
+        parts = device_parts(self.builder(theta))
+        return parts.cct_objects, parts.qs_objects
 
-    def backward(self, adj_final: numpy.ndarray) -> numpy.ndarray:
-        """dJ/dtheta for dJ/d(final state)."""
-        _, cct_grad, qs_grad = self.tracker.backward(adj_final)
-        model = self.tracker.model
-        ccts0, qss0 = self._magnet_params(self.theta)
-        # global CCT index -> CCT object of design 0 (zero-current CCTs carry no segments)
-        gl = {k: c for c, (d, k) in enumerate(model.cct_design)}
-        grad = numpy.zeros_like(self.theta)
-        for kth in range(self.theta.size):
-            h = self.fd_step * max(1.0, abs(self.theta[kth]))
-            tp, tm = self.theta.copy(), self.theta.copy()
+    def parameter_map(self, theta: numpy.ndarray):
+        """How each parameter moves the device's magnet parameters at theta (central
+        differences of the builder, step fd_step relative): per parameter k a list of
+        (CCT index, dI/dtheta_k, [dc_j/dtheta_k for the first four tilt coefficients]) and of
+        (QS index, dg/dtheta_k, ds/dtheta_k), indices in the device's flattened order.
+
+        Parameters enter through the CCT currents, the first four tilt angles of each CCT
+        (the tilt slots the adjoint carries) and the QS gradients; a parameter that moves
+        anything else the adjoint cannot see (a fifth tilt, the number of magnets or tilts)
+        raises ValueError.  Host only."""
+        theta = numpy.asarray(theta, numpy.float64)
+        ccts0, qss0 = self._magnet_params(theta)
+        out = []
+        for kth in range(theta.size):
+            h = self.fd_step * max(1.0, abs(theta[kth]))
+            tp, tm = theta.copy(), theta.copy()
             tp[kth] += h
             tm[kth] -= h
             cp, qp = self._magnet_params(tp)
             cm, qm = self._magnet_params(tm)
-            g = 0.0
+            if len(cp) != len(ccts0) or len(cm) != len(ccts0) or len(qp) != len(qss0) or len(qm) != len(qss0):
+                raise ValueError(f"parameter {kth} changes the number of CCT or QS magnets")
+            cct_terms, qs_terms = [], []
             for i, c0 in enumerate(ccts0):
+                nt = len(c0.tilt_angles)
+                if len(cp[i].tilt_angles) != nt or len(cm[i].tilt_angles) != nt:
+                    raise ValueError(f"parameter {kth} changes the number of tilt angles of CCT {i}")
+                for j in range(MAX_GRAD_TILTS, nt):
+                    if cp[i].tilt_angles[j] != cm[i].tilt_angles[j]:
+                        raise ValueError(f"parameter {kth} moves tilt angle {j + 1} of CCT {i}: gradients exist "
+                                         f"for the first {MAX_GRAD_TILTS} tilt angles only")
+                dI = (cp[i].current - cm[i].current) / (2 * h)
+                sh = math.sinh(c0.eta)
+                dc = [0.0] * MAX_GRAD_TILTS
+                for j, a0 in enumerate(c0.tilt_angles[:MAX_GRAD_TILTS]):
+                    da = (cp[i].tilt_angles[j] - cm[i].tilt_angles[j]) / (2 * h)
+                    if da != 0.0:
+                        dc[j] = -(math.pi / 180.0) / math.sin(math.radians(a0)) ** 2 / ((j + 1) * sh) * da
+                if dI != 0.0 or any(dc):
+                    cct_terms.append((i, dI, dc))
+            for i in range(len(qss0)):
+                dg = (qp[i].gradient - qm[i].gradient) / (2 * h)
+                ds = (qp[i].second_gradient - qm[i].second_gradient) / (2 * h)
+                if dg != 0.0 or ds != 0.0:
+                    qs_terms.append((i, dg, ds))
+            out.append((cct_terms, qs_terms))
+        return out, ccts0
+
+    def backward(self, adj_final: numpy.ndarray) -> numpy.ndarray:
+        """dJ/dtheta for dJ/d(final state) (parameters as in parameter_map)."""
+        pmap, ccts0 = self.parameter_map(self.theta)  # raises before any device work
+        _, cct_grad, qs_grad = self.tracker.backward(adj_final)
+        # design CCT index -> global CCT index (zero-current CCTs carry no segments)
+        gl = {k: c for c, (d, k) in enumerate(self.tracker.model.cct_design)}
+        grad = numpy.zeros_like(self.theta)
+        for kth, (cct_terms, qs_terms) in enumerate(pmap):
+            g = 0.0
+            for i, dI, dc in cct_terms:
                 if i not in gl:
                     continue
                 cg = cct_grad[gl[i]]
-                dI = (cp[i].current - cm[i].current) / (2 * h)
                 if dI != 0.0:
-                    g += cg[0] / c0.current * dI
-                sh = math.sinh(c0.eta)
-                for j, a0 in enumerate(c0.tilt_angles[:4]):
-                    da = (cp[i].tilt_angles[j] - cm[i].tilt_angles[j]) / (2 * h)
-                    if da == 0.0:
-                        continue
-                    r = math.radians(a0)
-                    dc_da = -(math.pi / 180.0) / math.sin(r) ** 2 / ((j + 1) * sh)
-                    g += cg[1 + j] * dc_da * da
-            for i, q0 in enumerate(qss0):
-                g += qs_grad[i, 0] * (qp[i].gradient - qm[i].gradient) / (2 * h)
-                g += qs_grad[i, 1] * (qp[i].second_gradient - qm[i].second_gradient) / (2 * h)
+                    g += cg[0] / ccts0[i].current * dI
+                for j in range(MAX_GRAD_TILTS):
+                    if dc[j] != 0.0:
+                        g += cg[1 + j] * dc[j]
+            for i, dg, ds in qs_terms:
+                g += qs_grad[i, 0] * dg + qs_grad[i, 1] * ds
             grad[kth] = g
         return grad
 

@@ -111,6 +111,7 @@ class CCT(Magnet, ApertureObject):
 
     def _device_parts(self, out: DeviceParts) -> None:
         out.cct_params.append(self.device_record())
+        out.cct_objects.append(self)
         # |I| <= 1e-6 gives no field (cct.py:254-255): no segments are generated
         out.cct_counts.append(0 if BaseUtils.equal(self.current, 0, err=1e-6) else self.total_disperse_number)
 

@@ -712,13 +712,13 @@ int cctamd_winding_tangents(const double* params, const int64_t* count_off, cons
     return hip_status(hipGetLastError());
 }
 
-int cctamd_track_rk4_taped(int precision, int ppl, const void* seg, const double* tile_org, const int32_t* work,
-                           int64_t nwork, const int64_t* design_wbase, const int32_t* design_nchunk,
+int cctamd_track_rk4_taped(int precision, int ppl, const void* seg, const double* tile_org, const double* seg_near,
+                           const int32_t* work, int64_t nwork, const int64_t* design_wbase, const int32_t* design_nchunk,
                            const int64_t* design_pt_off, int64_t ndesign, const double* qs,
                            const int64_t* design_qs_off, const double* bconst, double* state, int64_t nparticles,
                            const double* kq, const double* dt, const int64_t* nsteps, int64_t max_steps,
                            double* scratch, double* partial, double* tape, void* stream) {
-    if (!valid_common(precision, ppl, seg, tile_org, nwork, ndesign)) return CCTAMD_EINVAL;
+    if (!valid_common(precision, ppl, seg, tile_org, seg_near, nwork, ndesign)) return CCTAMD_EINVAL;
     if (nparticles < 0 || max_steps < 0) return CCTAMD_EINVAL;
     if (nparticles == 0 || max_steps == 0) return CCTAMD_OK;
     if (!state || !kq || !dt || !nsteps || !scratch || !tape || !design_wbase || !design_nchunk || !design_pt_off ||
@@ -726,14 +726,14 @@ int cctamd_track_rk4_taped(int precision, int ppl, const void* seg, const double
         return CCTAMD_EINVAL;
     if (nwork > 0 && (!work || !partial)) return CCTAMD_EINVAL;
     if (ceil_div(nparticles, kWavesPerWG) > 0x7fffffff) return CCTAMD_ERANGE;
-    const DeviceSet ds{precision, ppl, seg, tile_org, (const Work*)work, nwork, design_wbase, design_nchunk,
-                       design_pt_off, ndesign, qs, design_qs_off, bconst};
+    const DeviceSet ds{precision, ppl, seg, tile_org, seg_near, (const Work*)work, nwork, design_wbase,
+                       design_nchunk, design_pt_off, ndesign, qs, design_qs_off, bconst};
     return hip_status(launch_rk4_steps(ds, state, nparticles, kq, dt, nsteps, 0, max_steps, scratch, partial, nullptr,
                                        1, tape, (hipStream_t)stream));
 }
 
-int cctamd_track_rk4_adjoint(int precision, int ppl, const void* seg, const double* tile_org, const float* seg32,
-                             const float* seg_tan, const int32_t* tile_cct, const double* cct_coef, int64_t ntile,
+int cctamd_track_rk4_adjoint(int precision, int ppl, const void* seg, const double* tile_org,
+                             const double* seg_near, const float* seg32, const float* seg_tan, const int32_t* tile_cct, const double* cct_coef, int64_t ntile,
                              const int32_t* work, int64_t nwork, const int64_t* design_wbase,
                              const int32_t* design_nchunk, const int64_t* design_pt_off, int64_t ndesign,
                              const double* qs, const int64_t* design_qs_off, const double* bconst,
@@ -741,7 +741,7 @@ int cctamd_track_rk4_adjoint(int precision, int ppl, const void* seg, const doub
                              const double* dt, const int64_t* nsteps, int64_t max_steps, double* adj, double* tgrad,
                              double* qgrad, double* state_ws, double* scratch, double* partial, double* tape,
                              double* adj_ws, double* bs_pts, double* bs_wts, void* stream) {
-    if (!valid_common(precision, ppl, seg, tile_org, nwork, ndesign)) return CCTAMD_EINVAL;
+    if (!valid_common(precision, ppl, seg, tile_org, seg_near, nwork, ndesign)) return CCTAMD_EINVAL;
     if (nparticles < 0 || max_steps < 0 || ckpt_every < 0 || ntile < 0) return CCTAMD_EINVAL;
     if (nparticles == 0 || max_steps == 0) return CCTAMD_OK;
     const bool full_tape = ckpt_every == 0;  // tape of every step from cctamd_track_rk4_taped: no recompute
@@ -754,8 +754,8 @@ int cctamd_track_rk4_adjoint(int precision, int ppl, const void* seg, const doub
         return CCTAMD_EINVAL;
     hipStream_t st = (hipStream_t)stream;
     const int64_t P = nparticles;
-    const DeviceSet ds{precision, ppl, seg, tile_org, (const Work*)work, nwork, design_wbase, design_nchunk,
-                       design_pt_off, ndesign, qs, design_qs_off, bconst};
+    const DeviceSet ds{precision, ppl, seg, tile_org, seg_near, (const Work*)work, nwork, design_wbase,
+                       design_nchunk, design_pt_off, ndesign, qs, design_qs_off, bconst};
     const int64_t eblk = ceil_div(P, 256), rblk = ceil_div(P, kWavesPerWG);
     if (rblk > 0x7fffffff) return CCTAMD_ERANGE;
     hipError_t e;

@@ -144,11 +144,60 @@ __global__ void k_pack_fp32(const double* __restrict__ seg64, int64_t nseg, floa
 #pragma clang fp contract(fast)
 
 // ----------------------------------------------------- Biot-Savart core
+// 1/sqrt(x) in FP64 for a normal x > 0: the hardware estimate y refined by one
+// third-order step of the series (1 - e)^(-1/2) = 1 + e/2 + 3e^2/8 + O(e^3),
+// e = 1 - x y^2.  No special-value handling: a point on a segment midpoint
+// (x = 0) gives a NaN term, as the reference's norm(r)**(-3) times a zero
+// cross product does.
+__device__ __forceinline__ double rsqrt_f64(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = __builtin_fma(-x * y, y, 1.0);
+    return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
+}
+
+// FP64 pair terms of tile t (seg64 rows) for NP points, added to FP64 accumulators:
+// the term and summation order of k_bs_partial_f64.  The FP32 kernels take this
+// form for a tile within 2.5 tile extents of one of their points: there the
+// offset q = p - c from the tile origin is comparable with the distance to the
+// nearest segment, and rounding q (or m - c) to FP32 moves that segment's term by
+// ~1e-6 of itself, more than the whole FP32 sum may differ from the FP64 one.
+// t is wave-uniform in the scalar-load kernels (scalar loads) and row- or
+// group-uniform in the split-wave kernels (vector loads of a few addresses).
+template <int NP>
+__device__ __forceinline__ void tile_terms_f64(const double* __restrict__ seg64, int64_t t, const double (&px)[NP],
+                                               const double (&py)[NP], const double (&pz)[NP], double (&ax)[NP],
+                                               double (&ay)[NP], double (&az)[NP]) {
+    const double2* s = reinterpret_cast<const double2*>(seg64) + t * kTile * 4;
+#pragma unroll 4
+    for (int k = 0; k < kTile; ++k) {
+        const double2 k01 = s[4 * k], k2_ = s[4 * k + 1], m01 = s[4 * k + 2], m2_ = s[4 * k + 3];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const double rx = px[j] - m01.x, ry = py[j] - m01.y, rz = pz[j] - m2_.x;
+            const double r2 = rx * rx + ry * ry + rz * rz;
+            const double inv = rsqrt_f64(r2);
+            const double inv3 = inv * inv * inv;
+            ax[j] += (k01.y * rz - k2_.x * ry) * inv3;
+            ay[j] += (k2_.x * rx - k01.x * rz) * inv3;
+            az[j] += (k01.x * ry - k01.y * rx) * inv3;
+        }
+    }
+}
+
+// A tile is *near* a point when |p - c|^2 < kNearTile e^2 (c the tile origin, e
+// the tile extent, the largest |m - c| of its segments): within 2.5 extents.
+// Near tiles are summed in FP64 (tile_terms_f64) by every FP32 kernel, for the
+// whole wave (or lane group) when any of its points is near: a beam that leaves
+// the bore touches few tiles, so this costs little (0.2-0.7 % of point-tile
+// pairs on the AG-CCT section's recorded tracks).
+constexpr double kNearTile = 6.25;
+
 // FP32 terms on tiles re-centred at an FP64 origin; each tile's 32 terms are
-// summed in FP32 and the tile sums are accumulated in FP64.
+// summed in FP32 and the tile sums are accumulated in FP64; near tiles in FP64.
 template <int PPL>
 __global__ __launch_bounds__(kWG) void k_bs_partial_f32(const float* __restrict__ seg32,
                                                          const double* __restrict__ tile_org,
+                                                         const double* __restrict__ seg64,
                                                          const Work* __restrict__ work,
                                                          const double* __restrict__ pts, int64_t ptot,
                                                          double* __restrict__ partial) {
@@ -165,13 +214,21 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32(const float* __restrict_
     const int tend = w.tile_start + w.tile_count;
     for (int t = w.tile_start + wave; t < tend; t += kWavesPerWG) {
         const double ox = tile_org[4 * t], oy = tile_org[4 * t + 1], oz = tile_org[4 * t + 2];
+        const double near2 = kNearTile * tile_org[4 * t + 3];
         float qx[PPL], qy[PPL], qz[PPL], sx[PPL], sy[PPL], sz[PPL];
+        bool near = false;
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
-            qx[j] = (float)(px[j] - ox);
-            qy[j] = (float)(py[j] - oy);
-            qz[j] = (float)(pz[j] - oz);
+            const double dx = px[j] - ox, dy = py[j] - oy, dz = pz[j] - oz;
+            near = near || dx * dx + dy * dy + dz * dz < near2;
+            qx[j] = (float)dx;
+            qy[j] = (float)dy;
+            qz[j] = (float)dz;
             sx[j] = sy[j] = sz[j] = 0.0f;
+        }
+        if (__any(near)) {  // wave-uniform
+            tile_terms_f64<PPL>(seg64, t, px, py, pz, ax, ay, az);
+            continue;
         }
         const float4* s = reinterpret_cast<const float4*>(seg32) + (int64_t)t * kTile * 2;
 #pragma unroll 8
@@ -204,12 +261,9 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32(const float* __restrict_
 // K x q - C; and |q - m|^2 = (|q|^2 + |m|^2) - 2 q.m needs no difference
 // vector.  13 VALU slots per pair (add, 3 fma, rsq, 2 mul, 6 fma) instead of 17.
 // |q|^2 is formed in FP64 per tile.  Both rewrites cancel when the point is
-// close to the tile (|q| of the order of the tile extent e, the largest |m - c|
-// with c the mean midpoint): for |q| >= 2.5 e the factors (|q|^2 + e^2) / |r|^2
-// and |q| / |r| stay below 3.3, so a wave that has any lane with
-// |q|^2 < 6.25 e^2 (a point near or inside the windings) sums that tile in the
-// direct form (kl x r) |r|^-3 instead.
-constexpr double kNearTile = 6.25;
+// close to the tile (|q| of the order of the tile extent e): for |q| >= 2.5 e
+// the factors (|q|^2 + e^2) / |r|^2 and |q| / |r| stay below 3.3; nearer tiles
+// are summed in FP64 (see kNearTile).
 
 // Waves per workgroup of the scalar-load Biot-Savart kernels (K/C FP32 and FP64).
 // Scalar loads return out of order, so a wave waits for all of its segment loads
@@ -221,20 +275,10 @@ constexpr double kNearTile = 6.25;
 #endif
 constexpr int kBsWaves = CCTAMD_BS_WAVES;
 
-__device__ inline void tile_direct_term(const float4& a, const float4& b, const float4& c, float qx, float qy,
-                                        float qz, float& bx, float& by, float& bz) {
-    // m = -0.5 (-2m) is exact in FP32
-    const float rx = __builtin_fmaf(0.5f, b.z, qx), ry = __builtin_fmaf(0.5f, b.w, qy), rz = __builtin_fmaf(0.5f, c.x, qz);
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    const float inv = __builtin_amdgcn_rsqf(r2);
-    const float w3 = inv * inv * inv;
-    bx += (a.y * rz - a.z * ry) * w3;
-    by += (a.z * rx - a.x * rz) * w3;
-    bz += (a.x * ry - a.y * rx) * w3;
-}
 template <int PPL, int NW>
 __global__ __launch_bounds__(kWave * NW) void k_bs_partial_f32k(const float* __restrict__ seg32k,
                                                           const double* __restrict__ tile_org,
+                                                          const double* __restrict__ seg64,
                                                           const Work* __restrict__ work,
                                                           const double* __restrict__ pts, int64_t ptot,
                                                           double* __restrict__ partial) {
@@ -266,19 +310,8 @@ __global__ __launch_bounds__(kWave * NW) void k_bs_partial_f32k(const float* __r
             kx[j] = ky[j] = kz[j] = cx[j] = cy[j] = cz[j] = 0.0f;
         }
         const float4* s = reinterpret_cast<const float4*>(seg32k) + (int64_t)t * kTile * 3;
-        if (__any(near)) {  // wave-uniform: direct form for the whole wave on this tile
-#pragma unroll 1
-            for (int k = 0; k < kTile; ++k) {
-                const float4 a = s[3 * k], b = s[3 * k + 1], c = s[3 * k + 2];
-#pragma unroll
-                for (int j = 0; j < PPL; ++j) tile_direct_term(a, b, c, qx[j], qy[j], qz[j], kx[j], ky[j], kz[j]);
-            }
-#pragma unroll
-            for (int j = 0; j < PPL; ++j) {
-                ax[j] += (double)kx[j];
-                ay[j] += (double)ky[j];
-                az[j] += (double)kz[j];
-            }
+        if (__any(near)) {  // wave-uniform: FP64 terms for the whole wave on this tile
+            tile_terms_f64<PPL>(seg64, t, px, py, pz, ax, ay, az);
             continue;
         }
 #pragma unroll 8
@@ -367,20 +400,9 @@ __device__ __forceinline__ void kc_row_steps(const KCRow& g, const float (&qx)[2
     if constexpr (S + 1 < 16) kc_row_steps<S + 1>(g, qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
 }
 
-template <int S>
-__device__ __forceinline__ void direct_row_steps(const KCRow& g, const float (&qx)[2], const float (&qy)[2],
-                                                 const float (&qz)[2], float (&bx)[2], float (&by)[2],
-                                                 float (&bz)[2]) {
-    const KCRow h{make_float4(row_bcast<S>(g.a.x), row_bcast<S>(g.a.y), row_bcast<S>(g.a.z), 0.0f),
-                  make_float4(0.0f, 0.0f, row_bcast<S>(g.b.z), row_bcast<S>(g.b.w)),
-                  make_float4(row_bcast<S>(g.c.x), 0.0f, 0.0f, 0.0f)};
-#pragma unroll
-    for (int j = 0; j < 2; ++j) tile_direct_term(h.a, h.b, h.c, qx[j], qy[j], qz[j], bx[j], by[j], bz[j]);
-    if constexpr (S + 1 < 16) direct_row_steps<S + 1>(g, qx, qy, qz, bx, by, bz);
-}
-
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(6))) void k_bs_partial_f32k_row(const float* __restrict__ seg32k,
                                                              const double* __restrict__ tile_org,
+                                                             const double* __restrict__ seg64,
                                                              const Work* __restrict__ work,
                                                              const double* __restrict__ pts, int64_t ptot,
                                                              double* __restrict__ partial) {
@@ -406,7 +428,6 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(6))) void k
     }
     const int tend = w.tile_start + w.tile_count;
     const float4* src = reinterpret_cast<const float4*>(seg32k);
-    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     auto load = [&](int t, KCRow& g0, KCRow& g1) {  // segments gl and gl + 16 of tile t
         const float4* r0 = src + ((int64_t)t * kTile + gl) * 3;
         const float4* r1 = r0 + LG * 3;
@@ -433,15 +454,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(6))) void k
             qq[j] = (float)d2;
             kx[j] = ky[j] = kz[j] = cx[j] = cy[j] = cz[j] = 0.0f;
         }
-        if (__any(near)) {  // direct form for the active rows on this round
-            direct_row_steps<0>(g0, qx, qy, qz, kx, ky, kz);
-            direct_row_steps<0>(g1, qx, qy, qz, kx, ky, kz);
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                ax[j] += (double)kx[j];
-                ay[j] += (double)ky[j];
-                az[j] += (double)kz[j];
-            }
+        if (__any(near)) {  // FP64 terms for the active rows on this round, each row its own tile
+            tile_terms_f64<2>(seg64, t, px, py, pz, ax, ay, az);
         } else {
             kc_row_steps<0>(g0, qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
             kc_row_steps<0>(g1, qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
@@ -492,6 +506,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(6))) void k
 template <int G, int PL>
 __global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __restrict__ seg32k,
                                                               const double* __restrict__ tile_org,
+                                                              const double* __restrict__ seg64,
                                                               const Work* __restrict__ work,
                                                               const double* __restrict__ pts, int64_t ptot,
                                                               double* __restrict__ partial) {
@@ -544,19 +559,8 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __rest
             kx[j] = ky[j] = kz[j] = cx[j] = cy[j] = cz[j] = 0.0f;
         }
         const float4* r4 = lds[stream];
-        if (__any(near)) {  // direct form for this round
-#pragma unroll 1
-            for (int k = 0; k < kTile; ++k) {
-                const float4 a = r4[3 * k], b = r4[3 * k + 1], c = r4[3 * k + 2];
-#pragma unroll
-                for (int j = 0; j < PL; ++j) tile_direct_term(a, b, c, qx[j], qy[j], qz[j], kx[j], ky[j], kz[j]);
-            }
-#pragma unroll
-            for (int j = 0; j < PL; ++j) {
-                ax[j] += (double)kx[j];
-                ay[j] += (double)ky[j];
-                az[j] += (double)kz[j];
-            }
+        if (__any(near)) {  // FP64 terms for this round, each lane group its own tile (if any)
+            if (real) tile_terms_f64<PL>(seg64, t, px, py, pz, ax, ay, az);
             __syncthreads();
             continue;
         }
@@ -609,17 +613,6 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __rest
         const int c = k / PB, i = k % PB;
         out[(int64_t)(i * 3 + c) * w.nchunk] = ((red[0][c][i] + red[1][c][i]) + red[2][c][i]) + red[3][c][i];
     }
-}
-
-// 1/sqrt(x) in FP64 for a normal x > 0: the hardware estimate y refined by one
-// third-order step of the series (1 - e)^(-1/2) = 1 + e/2 + 3e^2/8 + O(e^3),
-// e = 1 - x y^2.  No special-value handling: a point on a segment midpoint
-// (x = 0) gives a NaN term, as the reference's norm(r)**(-3) times a zero
-// cross product does.
-__device__ __forceinline__ double rsqrt_f64(double x) {
-    const double y = __builtin_amdgcn_rsq(x);
-    const double e = __builtin_fma(-x * y, y, 1.0);
-    return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
 }
 
 // Plain FP64 terms on global-frame segments.
@@ -897,15 +890,15 @@ __global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __rest
 #pragma clang fp contract(fast)
 
 template <int PPL>
-hipError_t launch_bs(int precision, const void* seg, const double* tile_org, const Work* work, int64_t nwork,
-                     const double* pts, int64_t ptot, double* partial, hipStream_t st) {
+hipError_t launch_bs(int precision, const void* seg, const double* tile_org, const double* seg_near, const Work* work,
+                     int64_t nwork, const double* pts, int64_t ptot, double* partial, hipStream_t st) {
     if (nwork == 0) return hipSuccess;
     if (precision == 32)
         hipLaunchKernelGGL(k_bs_partial_f32<PPL>, dim3((unsigned)nwork), dim3(kWG), 0, st,
-                           (const float*)seg, tile_org, work, pts, ptot, partial);
+                           (const float*)seg, tile_org, seg_near, work, pts, ptot, partial);
     else if (precision == 33)
         hipLaunchKernelGGL((k_bs_partial_f32k<PPL, kBsWaves>), dim3((unsigned)nwork), dim3(kWave * kBsWaves), 0, st,
-                           (const float*)seg, tile_org, work, pts, ptot, partial);
+                           (const float*)seg, tile_org, seg_near, work, pts, ptot, partial);
     else
         hipLaunchKernelGGL((k_bs_partial_f64<PPL, kBsWaves>), dim3((unsigned)nwork), dim3(kWave * kBsWaves), 0, st,
                            (const double*)seg, work, pts, ptot, partial);
@@ -915,18 +908,19 @@ hipError_t launch_bs(int precision, const void* seg, const double* tile_org, con
 }  // namespace
 
 // ------------------------------------------------------- shared launchers
-bool valid_common(int precision, int ppl, const void* seg, const double* tile_org, int64_t nwork,
-                  int64_t ndesign) {
+bool valid_common(int precision, int ppl, const void* seg, const double* tile_org, const double* seg_near,
+                  int64_t nwork, int64_t ndesign) {
     if (precision != 32 && precision != 33 && precision != 64) return false;
     if (!valid_ppl(ppl) || (ppl < 0 && precision == 32)) return false;
     if (nwork < 0 || ndesign <= 0 || nwork > 0x7fffffff) return false;
     if (nwork > 0 && seg == nullptr) return false;
-    if (nwork > 0 && precision != 64 && tile_org == nullptr) return false;
+    if (nwork > 0 && precision != 64 && (tile_org == nullptr || seg_near == nullptr)) return false;
     return true;
 }
 
-hipError_t launch_bs_any(int precision, int ppl, const void* seg, const double* tile_org, const Work* work,
-                         int64_t nwork, const double* pts, int64_t ptot, double* partial, hipStream_t st) {
+hipError_t launch_bs_any(int precision, int ppl, const void* seg, const double* tile_org, const double* seg_near,
+                         const Work* work, int64_t nwork, const double* pts, int64_t ptot, double* partial,
+                         hipStream_t st) {
     if (ppl < 0) {
         if (nwork == 0) return hipSuccess;
         if (precision == 64) {
@@ -941,17 +935,38 @@ hipError_t launch_bs_any(int precision, int ppl, const void* seg, const double* 
         if (precision != 33) return hipErrorInvalidValue;
         if (ppl == -2)
             hipLaunchKernelGGL(k_bs_partial_f32k_row, dim3((unsigned)nwork), dim3(kWG), 0, st,
-                               (const float*)seg, tile_org, work, pts, ptot, partial);
+                               (const float*)seg, tile_org, seg_near, work, pts, ptot, partial);
         else
             hipLaunchKernelGGL((k_bs_partial_f32k_grp<8, 2>), dim3((unsigned)nwork), dim3(kWG), 0, st,
-                               (const float*)seg, tile_org, work, pts, ptot, partial);
+                               (const float*)seg, tile_org, seg_near, work, pts, ptot, partial);
         return hipGetLastError();
     }
     switch (ppl) {
-        case 1: return launch_bs<1>(precision, seg, tile_org, work, nwork, pts, ptot, partial, st);
-        case 2: return launch_bs<2>(precision, seg, tile_org, work, nwork, pts, ptot, partial, st);
-        default: return launch_bs<4>(precision, seg, tile_org, work, nwork, pts, ptot, partial, st);
+        case 1: return launch_bs<1>(precision, seg, tile_org, seg_near, work, nwork, pts, ptot, partial, st);
+        case 2: return launch_bs<2>(precision, seg, tile_org, seg_near, work, nwork, pts, ptot, partial, st);
+        default: return launch_bs<4>(precision, seg, tile_org, seg_near, work, nwork, pts, ptot, partial, st);
     }
+}
+
+int field_checked(const DeviceSet& ds, const double* points, int64_t npoints, double* partial, double* B,
+                  hipStream_t st) {
+    if (!valid_common(ds.precision, ds.ppl, ds.seg, ds.tile_org, ds.seg_near, ds.nwork, ds.ndesign))
+        return CCTAMD_EINVAL;
+    if (npoints < 0) return CCTAMD_EINVAL;
+    if (npoints == 0) return CCTAMD_OK;
+    if (!points || !B || !ds.design_wbase || !ds.design_nchunk || !ds.design_pt_off || !ds.design_qs_off ||
+        !ds.bconst)
+        return CCTAMD_EINVAL;
+    if (ds.nwork > 0 && (!ds.work || !partial)) return CCTAMD_EINVAL;
+    const int64_t blocks = ceil_div(npoints, kWavesPerWG);
+    if (blocks > 0x7fffffff) return CCTAMD_ERANGE;
+    hipError_t e = launch_bs_any(ds.precision, ds.ppl, ds.seg, ds.tile_org, ds.seg_near, ds.work, ds.nwork, points,
+                                 npoints, partial, st);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(k_field_reduce, dim3((unsigned)blocks), dim3(kWG), 0, st, ds.ppl, partial, ds.design_wbase,
+                       ds.design_nchunk, ds.design_pt_off, ds.ndesign, ds.qs, ds.design_qs_off, ds.bconst, points,
+                       npoints, B);
+    return hip_status(hipGetLastError());
 }
 
 hipError_t launch_rk4_steps(const DeviceSet& ds, double* state, int64_t ptot, const double* kq, const double* dt,
@@ -967,7 +982,8 @@ hipError_t launch_rk4_steps(const DeviceSet& ds, double* state, int64_t ptot, co
     for (int64_t step = step0; step < step1; ++step) {
         double* tp = tape ? tape + (step - step0) * 36 * ptot : nullptr;
 #define CCTAMD_STAGE(S)                                                                                          \
-    e = launch_bs_any(ds.precision, ds.ppl, ds.seg, ds.tile_org, ds.work, ds.nwork, scratch, ptot, partial, st); \
+    e = launch_bs_any(ds.precision, ds.ppl, ds.seg, ds.tile_org, ds.seg_near, ds.work, ds.nwork, scratch, ptot, \
+                      partial, st);                                                                              \
     if (e != hipSuccess) return e;                                                                               \
     hipLaunchKernelGGL(k_rk4_stage<S>, dim3((unsigned)rblocks), dim3(kWG), 0, st, ds.ppl, partial,               \
                        ds.design_wbase, ds.design_nchunk, ds.design_pt_off, ds.ndesign, ds.qs, ds.design_qs_off,  \
@@ -1095,44 +1111,35 @@ int64_t cctamd_plan(int64_t ndesign, const int64_t* design_ntile, const int64_t*
     return n;
 }
 
-int cctamd_bs_partial(int precision, int ppl, const void* seg, const double* tile_org, const int32_t* work,
-                      int64_t nwork, const double* points, int64_t npoints, double* partial, void* stream) {
-    if (!valid_common(precision, ppl, seg, tile_org, nwork, 1)) return CCTAMD_EINVAL;
+int cctamd_bs_partial(int precision, int ppl, const void* seg, const double* tile_org, const double* seg_near,
+                      const int32_t* work, int64_t nwork, const double* points, int64_t npoints, double* partial,
+                      void* stream) {
+    if (!valid_common(precision, ppl, seg, tile_org, seg_near, nwork, 1)) return CCTAMD_EINVAL;
     if (npoints < 0) return CCTAMD_EINVAL;
     if (nwork == 0 || npoints == 0) return CCTAMD_OK;
     if (!work || !points || !partial) return CCTAMD_EINVAL;
-    return hip_status(launch_bs_any(precision, ppl, seg, tile_org, (const Work*)work, nwork, points, npoints, partial,
-                                    (hipStream_t)stream));
+    return hip_status(launch_bs_any(precision, ppl, seg, tile_org, seg_near, (const Work*)work, nwork, points, npoints,
+                                    partial, (hipStream_t)stream));
 }
 
-int cctamd_bs_field(int precision, int ppl, const void* seg, const double* tile_org, const int32_t* work,
-                    int64_t nwork, const int64_t* design_wbase, const int32_t* design_nchunk,
+int cctamd_bs_field(int precision, int ppl, const void* seg, const double* tile_org, const double* seg_near,
+                    const int32_t* work, int64_t nwork, const int64_t* design_wbase, const int32_t* design_nchunk,
                     const int64_t* design_pt_off, int64_t ndesign, const double* qs, const int64_t* design_qs_off,
                     const double* bconst, const double* points, int64_t npoints, double* partial, double* B,
                     void* stream) {
-    if (!valid_common(precision, ppl, seg, tile_org, nwork, ndesign)) return CCTAMD_EINVAL;
-    if (npoints < 0) return CCTAMD_EINVAL;
-    if (npoints == 0) return CCTAMD_OK;
-    if (!points || !B || !design_wbase || !design_nchunk || !design_pt_off || !design_qs_off || !bconst)
-        return CCTAMD_EINVAL;
-    if (nwork > 0 && (!work || !partial)) return CCTAMD_EINVAL;
-    hipStream_t st = (hipStream_t)stream;
-    hipError_t e = launch_bs_any(precision, ppl, seg, tile_org, (const Work*)work, nwork, points, npoints, partial, st);
-    if (e != hipSuccess) return (int)e;
-    const int64_t blocks = ceil_div(npoints, kWavesPerWG);
-    if (blocks > 0x7fffffff) return CCTAMD_ERANGE;
-    hipLaunchKernelGGL(k_field_reduce, dim3((unsigned)blocks), dim3(kWG), 0, st, ppl, partial, design_wbase,
-                       design_nchunk, design_pt_off, ndesign, qs, design_qs_off, bconst, points, npoints, B);
-    return hip_status(hipGetLastError());
+    if (nwork > 0 && !partial) return CCTAMD_EINVAL;
+    const DeviceSet ds{precision, ppl, seg, tile_org, seg_near, (const Work*)work, nwork, design_wbase,
+                       design_nchunk, design_pt_off, ndesign, qs, design_qs_off, bconst};
+    return field_checked(ds, points, npoints, partial, B, (hipStream_t)stream);
 }
 
-int cctamd_track_rk4(int precision, int ppl, const void* seg, const double* tile_org, const int32_t* work,
-                     int64_t nwork, const int64_t* design_wbase, const int32_t* design_nchunk,
+int cctamd_track_rk4(int precision, int ppl, const void* seg, const double* tile_org, const double* seg_near,
+                     const int32_t* work, int64_t nwork, const int64_t* design_wbase, const int32_t* design_nchunk,
                      const int64_t* design_pt_off, int64_t ndesign, const double* qs, const int64_t* design_qs_off,
                      const double* bconst, double* state, int64_t nparticles, const double* kq, const double* dt,
                      const int64_t* nsteps, int64_t max_steps, double* scratch, double* partial, double* record,
                      int64_t record_every, void* stream) {
-    if (!valid_common(precision, ppl, seg, tile_org, nwork, ndesign)) return CCTAMD_EINVAL;
+    if (!valid_common(precision, ppl, seg, tile_org, seg_near, nwork, ndesign)) return CCTAMD_EINVAL;
     if (nparticles < 0 || max_steps < 0 || record_every <= 0) return CCTAMD_EINVAL;
     if (nparticles == 0) return CCTAMD_OK;
     if (!state || !kq || !dt || !nsteps || !scratch || !design_wbase || !design_nchunk || !design_pt_off ||
@@ -1140,8 +1147,8 @@ int cctamd_track_rk4(int precision, int ppl, const void* seg, const double* tile
         return CCTAMD_EINVAL;
     if (nwork > 0 && (!work || !partial)) return CCTAMD_EINVAL;
     if (ceil_div(nparticles, kWavesPerWG) > 0x7fffffff) return CCTAMD_ERANGE;
-    const DeviceSet ds{precision, ppl, seg, tile_org, (const Work*)work, nwork, design_wbase, design_nchunk,
-                       design_pt_off, ndesign, qs, design_qs_off, bconst};
+    const DeviceSet ds{precision, ppl, seg, tile_org, seg_near, (const Work*)work, nwork, design_wbase,
+                       design_nchunk, design_pt_off, ndesign, qs, design_qs_off, bconst};
     return hip_status(launch_rk4_steps(ds, state, nparticles, kq, dt, nsteps, 0, max_steps, scratch, partial, record,
                                        record_every, nullptr, (hipStream_t)stream));
 }

@@ -316,6 +316,7 @@ struct DeviceSet {  // one segment set + analytic magnets of D designs
     int precision, ppl;
     const void* seg;
     const double* tile_org;
+    const double* seg_near;  // FP64 rows for near tiles (precisions 32 and 33)
     const Work* work;
     int64_t nwork;
     const int64_t* design_wbase;
@@ -327,10 +328,15 @@ struct DeviceSet {  // one segment set + analytic magnets of D designs
     const double* bconst;
 };
 
-bool valid_common(int precision, int ppl, const void* seg, const double* tile_org, int64_t nwork,
-                  int64_t ndesign);
-hipError_t launch_bs_any(int precision, int ppl, const void* seg, const double* tile_org, const Work* work,
-                         int64_t nwork, const double* pts, int64_t ptot, double* partial, hipStream_t st);
+bool valid_common(int precision, int ppl, const void* seg, const double* tile_org, const double* seg_near,
+                  int64_t nwork, int64_t ndesign);
+hipError_t launch_bs_any(int precision, int ppl, const void* seg, const double* tile_org, const double* seg_near,
+                         const Work* work, int64_t nwork, const double* pts, int64_t ptot, double* partial,
+                         hipStream_t st);
+// Checked field evaluation (cctamd_bs_field): Biot-Savart pass into `partial` (nwork * 3 * PB
+// doubles), then the fixed-order reduction + QS + uniform fields into B. Returns a CCTAMD code.
+int field_checked(const DeviceSet& ds, const double* points, int64_t npoints, double* partial, double* B,
+                  hipStream_t st);
 // RK4 steps [step0, step1) of every particle (forward), optional record of the
 // state every `record_every` steps (row (step+1)/record_every) and optional
 // tape of the stage points, velocities and fields relative to step0

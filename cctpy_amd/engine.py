@@ -244,12 +244,13 @@ class DeviceModel:
         return p
 
     def _seg(self, precision: int):
+        """(segment rows, tile origins, FP64 rows for near tiles) of a precision."""
         if precision == 64:
-            return self.seg64, None
+            return self.seg64, None, None
         if precision in (32, 33):
             if self.seg32 is None:
                 raise ValueError("model was built without FP32 tiles")
-            return (self.seg32 if precision == 32 else self.seg32k), self.tile_org
+            return (self.seg32 if precision == 32 else self.seg32k), self.tile_org, self.seg64
         raise ValueError("precision must be 32, 33 or 64")
 
     # ------------------------------------------------------------- field
@@ -269,8 +270,8 @@ class DeviceModel:
         soa = torch.from_numpy(numpy.ascontiguousarray(pts.T)).to(dev)
         out = torch.empty_like(soa)
         partial = torch.empty(max(plan.nwork * 3 * points_per_block(ppl), 1), dtype=torch.float64, device=dev)
-        seg, org = self._seg(precision)
-        N.check(N.lib.cctamd_bs_field(precision, ppl, _p(seg), _p(org), _p(plan.work), plan.nwork, _p(plan.wbase),
+        seg, org, near = self._seg(precision)
+        N.check(N.lib.cctamd_bs_field(precision, ppl, _p(seg), _p(org), _p(near), _p(plan.work), plan.nwork, _p(plan.wbase),
                                       _p(plan.nchunk), _p(plan.design_pt_off), self.ndesign, _p(self.qs),
                                       _p(self.design_qs_off), _p(self.bconst), _p(soa), plan.npt, _p(partial),
                                       _p(out), _stream()), "cctamd_bs_field")
@@ -305,8 +306,8 @@ class DeviceModel:
         scratch = torch.empty((14, plan.npt), dtype=torch.float64, device=dev)
         partial = torch.empty(max(plan.nwork * 3 * points_per_block(ppl), 1), dtype=torch.float64, device=dev)
         rec = torch.empty((max_steps + 1, 6, plan.npt), dtype=torch.float64, device=dev) if record else None
-        seg, org = self._seg(precision)
-        N.check(N.lib.cctamd_track_rk4(precision, ppl, _p(seg), _p(org), _p(plan.work), plan.nwork, _p(plan.wbase),
+        seg, org, near = self._seg(precision)
+        N.check(N.lib.cctamd_track_rk4(precision, ppl, _p(seg), _p(org), _p(near), _p(plan.work), plan.nwork, _p(plan.wbase),
                                        _p(plan.nchunk), _p(plan.design_pt_off), self.ndesign, _p(self.qs),
                                        _p(self.design_qs_off), _p(self.bconst), _p(soa), plan.npt, _p(kq_t),
                                        _p(dt_t), _p(ns_t), max_steps, _p(scratch), _p(partial), _p(rec), 1,
@@ -325,8 +326,8 @@ class DeviceModel:
                       nsteps: torch.Tensor, max_steps: int, scratch: torch.Tensor, partial: torch.Tensor,
                       record: Optional[torch.Tensor] = None, record_every: int = 1) -> None:
         """In-place RK4 tracking of device-resident SoA state (6, P); nothing crosses PCIe."""
-        seg, org = self._seg(precision)
-        N.check(N.lib.cctamd_track_rk4(precision, plan.ppl, _p(seg), _p(org), _p(plan.work), plan.nwork,
+        seg, org, near = self._seg(precision)
+        N.check(N.lib.cctamd_track_rk4(precision, plan.ppl, _p(seg), _p(org), _p(near), _p(plan.work), plan.nwork,
                                        _p(plan.wbase), _p(plan.nchunk), _p(plan.design_pt_off), self.ndesign,
                                        _p(self.qs), _p(self.design_qs_off), _p(self.bconst), _p(state), plan.npt,
                                        _p(kq), _p(dt), _p(nsteps), int(max_steps), _p(scratch), _p(partial),
@@ -335,16 +336,16 @@ class DeviceModel:
     def field_tensors(self, plan: "Plan", precision: int, points: torch.Tensor, partial: torch.Tensor,
                       out: torch.Tensor) -> None:
         """B (3, P) at device points (3, P): Biot-Savart pass + fixed-order reduction + QS + uniform."""
-        seg, org = self._seg(precision)
-        N.check(N.lib.cctamd_bs_field(precision, plan.ppl, _p(seg), _p(org), _p(plan.work), plan.nwork,
+        seg, org, near = self._seg(precision)
+        N.check(N.lib.cctamd_bs_field(precision, plan.ppl, _p(seg), _p(org), _p(near), _p(plan.work), plan.nwork,
                                       _p(plan.wbase), _p(plan.nchunk), _p(plan.design_pt_off), self.ndesign,
                                       _p(self.qs), _p(self.design_qs_off), _p(self.bconst), _p(points), plan.npt,
                                       _p(partial), _p(out), _stream()), "cctamd_bs_field")
 
     def bs_partial_tensors(self, plan: "Plan", precision: int, points: torch.Tensor, partial: torch.Tensor) -> None:
         """One launch of the split-K Biot-Savart kernel on device points (3, P)."""
-        seg, org = self._seg(precision)
-        N.check(N.lib.cctamd_bs_partial(precision, plan.ppl, _p(seg), _p(org), _p(plan.work), plan.nwork,
+        seg, org, near = self._seg(precision)
+        N.check(N.lib.cctamd_bs_partial(precision, plan.ppl, _p(seg), _p(org), _p(near), _p(plan.work), plan.nwork,
                                         _p(points), plan.npt, _p(partial), _stream()), "cctamd_bs_partial")
 
     def segments64(self) -> numpy.ndarray:

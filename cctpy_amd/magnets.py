@@ -161,6 +161,9 @@ class DeviceParts:
         self.qs: List[numpy.ndarray] = []
         self.bconst = numpy.zeros(3)
         self.foreign: List[Magnet] = []
+        # the magnet objects behind cct_params and qs, same order (parameter gradients map onto them)
+        self.cct_objects: List[Magnet] = []
+        self.qs_objects: List[Magnet] = []
 
     def spec(self):
         from .engine import DesignSpec
@@ -289,6 +292,7 @@ class QS(Magnet, ApertureObject):
 
     def _device_parts(self, out: DeviceParts) -> None:
         out.qs.append(self.to_numpy_array())
+        out.qs_objects.append(self)
 
     @staticmethod
     def as_qs(anything) -> "QS":

@@ -15,6 +15,11 @@
  *   seg32k     float[S][12]  {kl, kl x (m-c), -2(m-c), |m-c|^2, 0, 0}
  *   tile_org   double[S/CCTAMD_TILE][4]  {c, e2}: FP64 tile origin (mean of the
  *              tile's current-carrying midpoints) and extent max |m-c|^2
+ *   seg_near   the seg64 rows of the same set, passed beside the FP32 tiles: the FP32
+ *              paths (precisions 32 and 33) sum a tile in FP64 from these rows when a
+ *              point lies within 2.5 tile extents of its origin (|p-c|^2 < 6.25 e2),
+ *              where FP32 terms would lose ~1e-6 of the nearest segment's share;
+ *              ignored (may be NULL) at precision 64
  *   qs         double[Q][16] {O, XI, YI, ZI, length, gradient, second_gradient,
  *              aperture_radius} (the reference's QS record)
  *   bconst     double[D][4]  uniform field per design (sum of uniform magnets)
@@ -105,12 +110,12 @@ int64_t cctamd_plan(int64_t ndesign, const int64_t *design_ntile, const int64_t 
  *   precision: 64 (seg64, FP64 sum), 32 (seg32 + tile_org: FP32 terms of the
  *              re-centred tile, FP64 accumulation across tiles) or 33 (seg32k +
  *              tile_org: the same with per-tile sums K = sum w kl, C = sum w kl x m,
- *              B_tile = K x q - C and |q-m|^2 = (|q|^2 + |m|^2) - 2 q.m; for points away
- *              from the windings, e.g. in a bore).
+ *              B_tile = K x q - C and |q-m|^2 = (|q|^2 + |m|^2) - 2 q.m); at both FP32
+ *              precisions tiles near a point are summed in FP64 from seg_near.
  *   partial: device workspace of nwork*3*PB doubles (PB = points per block); design d's
  *            partials start at design_wbase[d]*3*PB, laid out [block][point][component][chunk].
  */
-int cctamd_bs_field(int precision, int ppl, const void *seg, const double *tile_org,
+int cctamd_bs_field(int precision, int ppl, const void *seg, const double *tile_org, const double *seg_near,
                     const int32_t *work, int64_t nwork, const int64_t *design_wbase,
                     const int32_t *design_nchunk, const int64_t *design_pt_off, int64_t ndesign,
                     const double *qs, const int64_t *design_qs_off, const double *bconst,
@@ -123,7 +128,7 @@ int cctamd_bs_field(int precision, int ppl, const void *seg, const double *tile_
  * cctamd_bs_field / cctamd_track_rk4, exported for timing and for callers
  * that reduce partials themselves.
  */
-int cctamd_bs_partial(int precision, int ppl, const void *seg, const double *tile_org,
+int cctamd_bs_partial(int precision, int ppl, const void *seg, const double *tile_org, const double *seg_near,
                       const int32_t *work, int64_t nwork, const double *points, int64_t npoints,
                       double *partial, void *stream);
 
@@ -140,7 +145,7 @@ int cctamd_bs_partial(int precision, int ppl, const void *seg, const double *til
  *            after every record_every-th step (row k = state after k*record_every
  *            steps; rows past a particle's own step count are left untouched).
  */
-int cctamd_track_rk4(int precision, int ppl, const void *seg, const double *tile_org,
+int cctamd_track_rk4(int precision, int ppl, const void *seg, const double *tile_org, const double *seg_near,
                      const int32_t *work, int64_t nwork, const int64_t *design_wbase,
                      const int32_t *design_nchunk, const int64_t *design_pt_off, int64_t ndesign,
                      const double *qs, const int64_t *design_qs_off, const double *bconst,
@@ -170,7 +175,7 @@ int cctamd_winding_tangents(const double *params, const int64_t *count_off, cons
  * recompute of the checkpointed mode.
  */
 int cctamd_track_rk4_taped(int precision, int ppl, const void *seg, const double *tile_org,
-                           const int32_t *work, int64_t nwork, const int64_t *design_wbase,
+                           const double *seg_near, const int32_t *work, int64_t nwork, const int64_t *design_wbase,
                            const int32_t *design_nchunk, const int64_t *design_pt_off, int64_t ndesign,
                            const double *qs, const int64_t *design_qs_off, const double *bconst,
                            double *state, int64_t nparticles, const double *kq, const double *dt,
@@ -198,7 +203,7 @@ int cctamd_track_rk4_taped(int precision, int ppl, const void *seg, const double
  *     tape[ckpt_every][36][P] (or the full tape), adj_ws[15][P], bs_pts[3][P], bs_wts[3][P].
  */
 int cctamd_track_rk4_adjoint(int precision, int ppl, const void *seg, const double *tile_org,
-                             const float *seg32, const float *seg_tan, const int32_t *tile_cct,
+                             const double *seg_near, const float *seg32, const float *seg_tan, const int32_t *tile_cct,
                              const double *cct_coef, int64_t ntile, const int32_t *work, int64_t nwork,
                              const int64_t *design_wbase, const int32_t *design_nchunk,
                              const int64_t *design_pt_off, int64_t ndesign, const double *qs,

@@ -30,6 +30,16 @@ def _problem(n_particles, n, length, step, precision, ckpt):
     return prob, beam
 
 
+def per_parameter_error(grad, fd, floor_frac=1e-3):
+    """|g_k - fd_k| / max(|fd_k|, floor) per parameter, floor = floor_frac * max |fd|: a small
+    component is held to its own size (down to 1/1000 of the largest), not to the largest one's.
+    The parameters mix T/m, degrees and amperes, so a single max-normalised figure would hide a
+    wrong small component."""
+    grad, fd = np.asarray(grad), np.asarray(fd)
+    floor = floor_frac * np.abs(fd).max()
+    return np.abs(grad - fd) / np.maximum(np.abs(fd), floor)
+
+
 def _fd(prob, theta, J, rel_h=1e-6):
     g = np.zeros_like(theta)
     for k in range(theta.size):
@@ -50,8 +60,9 @@ def test_adjoint_linear_functional_vs_fd():
     w = rng.normal(size=final.shape) * np.array([1.0, 1.0, 1.0, 1e-8, 1e-8, 1e-8])
     grad = prob.backward(w)
     fd = _fd(prob, theta, lambda f: float(np.sum(w * f)))
-    scale = np.abs(fd).max()
-    assert np.max(np.abs(grad - fd)) / scale < 1e-4, (grad, fd)
+    err = per_parameter_error(grad, fd)
+    print("per-parameter relative error:", np.array2string(err, precision=2))
+    assert err.max() < 1e-4, (grad, fd, err)
 
 
 def test_adjoint_beam_spot_objective_torch():
@@ -79,7 +90,9 @@ def test_adjoint_beam_spot_objective_torch():
     J.backward()
     fd = _fd(prob, theta.detach().numpy().copy(), J_np)
     g = theta.grad.numpy()
-    assert np.max(np.abs(g - fd)) / np.abs(fd).max() < 1e-4, (g, fd)
+    err = per_parameter_error(g, fd)
+    print("per-parameter relative error:", np.array2string(err, precision=2))
+    assert err.max() < 1e-4, (g, fd, err)
 
 
 @pytest.mark.parametrize("ppl", [1, -2])
@@ -127,7 +140,8 @@ def test_adjoint_population_current_gradients_vs_fd(ppl):
         fd.append((J[0] - J[1]) / (2 * h))
         got.append(cct_grad[c, 0] / cur)
     got, fd = np.array(got), np.array(fd)
-    assert np.max(np.abs(got - fd)) / np.abs(fd).max() < 1e-4, (got, fd)
+    err = per_parameter_error(got, fd)
+    assert err.max() < 1e-4, (got, fd, err)
 
 
 def test_adjoint_vs_fp64_cpu_oracle_gradient():
@@ -159,7 +173,57 @@ def test_adjoint_vs_fp64_cpu_oracle_gradient():
         tp[k] += h
         tm[k] -= h
         fd[k] = (J_cpu(tp) - J_cpu(tm)) / (2 * h)
-    assert np.max(np.abs(grad - fd)) / np.abs(fd).max() < 1e-4, (grad, fd)
+    err = per_parameter_error(grad, fd)
+    print("per-parameter relative error:", np.array2string(err, precision=2))
+    assert err.max() < 1e-4, (grad, fd, err)
+
+
+def test_adjoint_beam_spot_objective_vs_cpu_oracle():
+    """The beam-spot objective J = rms x + rms y + |D| (D = least-squares slope of x against
+    delta) at the end plane, differentiated by the GPU adjoint (FP32 Biot-Savart, checkpoints
+    every 20 steps, J and its seed through torch autograd) against central differences of J
+    evaluated on final states of the FP64 CPU oracle, per parameter of the 10-parameter vector
+    (reduced size: 60 segments per winding, 12 protons, 1 m in 5 mm steps)."""
+    from oracle import c_oracle as C, cct_oracle as O, workload_builder as WB
+    n, length, step, s0 = 60, 1.0, 5e-3, 0.4
+    beam = W.gaussian_beam(np.random.default_rng(29), 12, 0.04)
+    prob, _ = _problem(12, n, length, step, 33, 20)
+    prob.particles = lambda bl: W.beam_particles(W.Beam(beam, s0, 250.0), bl)
+    bl0 = W.build_beamline(W.section_ops(W.section_theta0(), n, 0.0))
+    ip = ParticleFactory.create_proton_along(bl0, s0 + length, 250.0)
+    fr = ip.get_natural_coordinate_system()
+    args = (ip.position.to_list(), fr.XI.to_list(), fr.YI.to_list(), beam[:, 4])
+    theta = torch.tensor(W.section_theta0(), requires_grad=True)
+    J = W.beam_spot_objective(adjoint.track_fn(theta, prob), *args)
+    J.backward()
+    g = theta.grad.numpy().copy()
+    th0 = theta.detach().numpy().copy()
+    # the end plane of the oracle's own ideal particle (same orbit: theta moves no geometry)
+    ref0 = WB.build_beamline(W.section_ops(th0, n, 0.0))
+    oip = O.create_proton_along(ref0.traj, s0 + length, 250.0)
+    ofr = oip.natural_frame()
+    assert np.max(np.abs(np.array(oip.position) - np.array(args[0]))) < 1e-12
+
+    def J_cpu(th):
+        bl = WB.build_beamline(W.section_ops(th, n, 0.0))
+        st, kq, speed = WB.beam_state(beam, s0, 250.0, bl)
+        sd = [O.rk4_steps(length, step, s) for s in speed]
+        out = C.track(O.beamline_arrays(bl.magnets), st, kq, np.array([d for _, d in sd]),
+                      np.array([s for s, _ in sd], np.int64))
+        return float(W.beam_spot_objective(torch.from_numpy(out), list(oip.position), list(ofr.XI), list(ofr.YI),
+                                           beam[:, 4]))
+
+    fd = np.zeros_like(th0)
+    for k in range(th0.size):
+        h = 1e-6 * max(abs(th0[k]), 1.0)
+        tp, tm = th0.copy(), th0.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd[k] = (J_cpu(tp) - J_cpu(tm)) / (2 * h)
+    assert abs(J_cpu(th0) - float(J)) <= 1e-5 * abs(float(J))
+    err = per_parameter_error(g, fd)
+    print("per-parameter relative error:", np.array2string(err, precision=2))
+    assert err.max() < 1e-4, (g, fd, err)
 
 
 def test_adjoint_split_wave_blocks_match_full_blocks():
@@ -187,7 +251,9 @@ def test_adjoint_split_wave_blocks_match_full_blocks():
         a0, cg, _ = tr.backward(w)
         res.append((final, a0, cg))
     for final, a0, cg in res[1:]:
-        assert np.max(np.abs(final[:, :3] - res[0][0][:, :3])) < 1e-9
+        # the shapes decide near tiles (summed in FP64) over different lane sets (a wave or a
+        # 16-lane group), so a few tiles are FP32 K/C sums in one shape and FP64 in the other
+        assert np.max(np.abs(final[:, :3] - res[0][0][:, :3])) < 1e-8
         assert np.max(np.abs(a0 - res[0][1])) <= 1e-6 * np.max(np.abs(res[0][1]))
         assert np.max(np.abs(cg - res[0][2])) <= 1e-5 * np.max(np.abs(res[0][2]))
 

@@ -91,7 +91,9 @@ def test_plan_rejects_bad_arguments():
     neg = np.array([-1], np.int64)
     assert N.lib.cctamd_plan(1, p(neg), p(z), p(z), p(z), 2, 10, 4, None, None, None) == -22
     # split-wave blocks exist for precisions 33 and 64, not for the re-centred FP32 kernel (32)
-    assert N.lib.cctamd_bs_partial(32, -2, p(z), p(z), p(z), 1, p(z), 1, p(z), None) == -22
+    assert N.lib.cctamd_bs_partial(32, -2, p(z), p(z), p(z), p(z), 1, p(z), 1, p(z), None) == -22
+    # the FP32 paths read FP64 rows for near tiles: a NULL seg_near is rejected before any launch
+    assert N.lib.cctamd_bs_partial(33, 1, p(z), p(z), None, p(z), 1, p(z), 1, p(z), None) == -22
 
 
 def test_adjoint_requires_fp32_tile_origins():
@@ -99,7 +101,7 @@ def test_adjoint_requires_fp32_tile_origins():
     tile_org is rejected on the host (before any launch) also for precision 64."""
     buf = np.zeros(64)
     q = buf.ctypes.data_as(ctypes.c_void_p)
-    args = [64, 2, q, None, q, q, q, q, 1, q, 1, q, q, q, 1, q, q, q, q, 4, 1, q, q, q, 1,
+    args = [64, 2, q, None, None, q, q, q, q, 1, q, 1, q, q, q, 1, q, q, q, q, 4, 1, q, q, q, 1,
             q, q, q, q, q, q, q, q, q, q, None]
     assert N.lib.cctamd_track_rk4_adjoint(*args) == -22
 
@@ -211,3 +213,40 @@ def test_block_shape_for_particle_counts():
     assert block_shape_for([16], 32) == 1      # no split-wave kernel for the re-centred FP32 field path
     assert block_shape_for([65, 1], 64) == 2
     assert block_shape_for([], 64) == -4       # nothing to track: any valid code
+
+
+def _nested_design(th):
+    """C05-style section whose two CCT layers sit in a CombinedMagnet placed before the QS; the
+    CCTs carry five tilt angles, th[1] moves the second and th[2] (if given) the fifth."""
+    from cctpy_amd.magnets import CombinedMagnet
+    tilts = [30.0, th[1], 90.0, 90.0, th[2] if len(th) > 2 else 60.0]
+    bl = (Beamline.set_start_point(P2.origin()).first_drift(P2.x_direct(), 1.0)
+          .append_qs(0.27, th[0], 0.0, 0.1).append_drift(1.0)
+          .append_dipole_cct(1.0, 0.1, 0.12, 45.0, tilts, 60, 10000.0).append_drift(1.0))
+    ccts = [m for m in bl.magnets if isinstance(m, CCT)]
+    bl.magnets = [CombinedMagnet(*ccts)] + [m for m in bl.magnets if not isinstance(m, CCT)]
+    return bl
+
+
+def test_design_parameter_map_follows_the_device_order_and_guards_tilts():
+    """DesignProblem maps parameters onto the magnets in the order the device holds them (CCTs
+    nested in a CombinedMagnet included), and refuses a parameter that moves a tilt angle beyond
+    the four the adjoint carries."""
+    from cctpy_amd.adjoint import MAX_GRAD_TILTS, DesignProblem
+    from cctpy_amd.magnets import device_parts
+    prob = DesignProblem(_nested_design, lambda bl: [], 1.0, 0.01)
+    parts = device_parts(_nested_design([5.0, 80.0]))
+    assert [type(m).__name__ for m in parts.cct_objects] == ["CCT", "CCT"] and len(parts.qs_objects) == 1
+    pmap, ccts = prob.parameter_map(np.array([5.0, 80.0]))
+    (c0, q0), (c1, q1) = pmap
+    assert c0 == [] and q0 == [(0, pytest.approx(1.0, rel=1e-9), 0.0)]
+    assert q1 == [] and [i for i, _, _ in c1] == [0, 1]
+    for i, dI, dc in c1:
+        assert dI == 0.0 and dc[0] == 0.0 and dc[2] == 0.0 and dc[3] == 0.0
+        # dc/dalpha = -(pi/180) / sin^2(alpha) / (2 sinh(eta)) for the second tilt
+        a = ccts[i].tilt_angles[1]
+        want = -(math.pi / 180.0) / math.sin(math.radians(a)) ** 2 / (2 * math.sinh(ccts[i].eta))
+        assert dc[1] == pytest.approx(want * (1.0 if a == 80.0 else -1.0), rel=1e-6)
+    assert MAX_GRAD_TILTS == 4
+    with pytest.raises(ValueError, match="tilt angle 5"):
+        prob.parameter_map(np.array([5.0, 80.0, 60.0]))
