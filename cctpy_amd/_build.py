@@ -6,7 +6,7 @@ import shutil
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", f) for f in ("cctamd.hip", "adjoint.hip", "beam.hip")]
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("cctamd.hip", "adjoint.hip", "beam.hip", "noalloc.hip")]
 LIB = os.path.join(HERE, "libcctamd.so")
 ARCH = "gfx950"
 

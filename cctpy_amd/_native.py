@@ -15,6 +15,7 @@ I64 = ctypes.c_int64
 INT = ctypes.c_int
 
 TILE = 32
+NOALLOC_PARTIAL_DOUBLES = 2 * 1024 * 1024
 CCT_PARAMS = 32
 MAX_TILTS = 8
 WORK_INTS = 8
@@ -48,6 +49,8 @@ def _load() -> ctypes.CDLL:
     lib.cctamd_bs_partial.restype = INT
     lib.cctamd_bs_field.argtypes = [INT, INT, P, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P]
     lib.cctamd_bs_field.restype = INT
+    lib.cctamd_bs_field_noalloc.argtypes = [INT, INT, P, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P]
+    lib.cctamd_bs_field_noalloc.restype = INT
     lib.cctamd_track_rk4.argtypes = [INT, INT, P, P, P, P, I64, P, P, P, I64, P, P, P, P, I64, P, P, P, I64, P, P,
                                      P, I64, P]
     lib.cctamd_track_rk4.restype = INT
@@ -85,5 +88,5 @@ EXPORTED = (
     "cctamd_version", "cctamd_error_string", "cctamd_winding_generate", "cctamd_winding_points",
     "cctamd_pack_fp32", "cctamd_plan", "cctamd_bs_partial", "cctamd_bs_field", "cctamd_track_rk4",
     "cctamd_winding_tangents", "cctamd_track_rk4_taped", "cctamd_track_rk4_adjoint", "cctamd_phase_space",
-    "cctamd_beam_objective_seed",
+    "cctamd_beam_objective_seed", "cctamd_bs_field_noalloc",
 )

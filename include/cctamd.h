@@ -122,6 +122,24 @@ int cctamd_bs_field(int precision, int ppl, const void *seg, const double *tile_
                     const double *points, int64_t npoints, double *partial, double *B, void *stream);
 
 /*
+ * cctamd_bs_field without the partial workspace.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Arguments as for cctamd_bs_field
+ * minus `partial`; the partial sums live in a static device array of
+ * CCTAMD_NOALLOC_PARTIAL_DOUBLES doubles (16 MiB) that each GPU holds until the process
+ * ends.  A plan needing more (nwork * 3 * PB doubles, PB = points per block) returns
+ * CCTAMD_ERANGE before any launch.  Calls on one GPU share its array: calls that overlap
+ * on one GPU (other streams or threads) race.  Allocates and frees nothing.
+ */
+#define CCTAMD_NOALLOC_PARTIAL_DOUBLES (2 * 1024 * 1024)
+int cctamd_bs_field_noalloc(int precision, int ppl, const void *seg, const double *tile_org,
+                            const double *seg_near, const int32_t *work, int64_t nwork,
+                            const int64_t *design_wbase, const int32_t *design_nchunk,
+                            const int64_t *design_pt_off, int64_t ndesign, const double *qs,
+                            const int64_t *design_qs_off, const double *bconst, const double *points,
+                            int64_t npoints, double *B, void *stream);
+
+/*
  * The split-K Biot-Savart pass alone: for every work item, the sum over its
  * tiles for each point of its block, written in the partial layout of
  * cctamd_bs_field (no reduction).  The building block of
