@@ -779,3 +779,128 @@ def beamline_arrays(magnets: Sequence) -> Dict[str, numpy.ndarray]:
         uni=numpy.ascontiguousarray(numpy.array([u.b for u in unis], dtype=numpy.float64).reshape(len(unis), 3)),
         order=numpy.ascontiguousarray(numpy.array(order, dtype=numpy.int64).reshape(len(order), 2)),
     )
+
+
+# ------------------------------------------------------ phase ellipses
+# BaseUtils.Ellipse (final_code/packages/base_utils.py:446-601): the ellipse
+# A x^2 + B x y + C y^2 = D, walked counter-clockwise from its +x axis point in
+# angle steps of 0.05 deg, chord lengths summed.
+def ellipse_point_at(A: float, B: float, C: float, D: float, theta: float) -> Vec2:
+    """base_utils.py:480-524: the ray from the origin at angle theta meets the ellipse."""
+    x, y = 0.0, 0.0
+    while theta < 0:
+        theta += 2 * math.pi
+    while theta > 2 * math.pi:
+        theta -= 2 * math.pi
+    if equal(theta, 0) or equal(theta, 2 * math.pi):
+        x, y = math.sqrt(D / A), 0
+    if equal(theta, math.pi):
+        x, y = -math.sqrt(D / A), 0
+    if 0 < theta < math.pi:
+        t = 1 / math.tan(theta)
+        y = math.sqrt(D / (A * t * t + B * t + C))
+        x = t * y
+    if math.pi < theta < 2 * math.pi:
+        theta -= math.pi
+        t = 1 / math.tan(theta)
+        y = -math.sqrt(D / (A * t * t + B * t + C))
+        x = t * y
+    return (x, y)
+
+
+def ellipse_circumference(A: float, B: float, C: float, D: float) -> float:
+    """base_utils.py:529-551: sum of 14,400 chords at equal angle steps."""
+    num = 3600 * 4
+    c = 0.0
+    for i in range(num):
+        c += p2_len(p2_sub(ellipse_point_at(A, B, C, D, 2.0 * math.pi / float(num) * (i + 1)),
+                           ellipse_point_at(A, B, C, D, 2.0 * math.pi / float(num) * i)))
+    return c
+
+
+def ellipse_point_after(A: float, B: float, C: float, D: float, length: float) -> Vec2:
+    """base_utils.py:553-573: walk `length` along the edge in 0.05 deg angle steps."""
+    step = deg_to_rad(0.05)
+    theta = 0.0
+    while length > 0.0:
+        length -= p2_len(p2_sub(ellipse_point_at(A, B, C, D, theta + step), ellipse_point_at(A, B, C, D, theta)))
+        theta += step
+    return ellipse_point_at(A, B, C, D, theta)
+
+
+def ellipse_edge_points(A: float, B: float, C: float, D: float, num: int) -> List[Vec2]:
+    """base_utils.py:575-584 uniform_distribution_points_along_edge."""
+    c = ellipse_circumference(A, B, C, D)
+    return [ellipse_point_after(A, B, C, D, c / num * i) for i in range(num)]
+
+
+def phase_ellipse(plane: str, amax: float, apmax: float, delta: float, number: int) -> List[Tuple]:
+    """(x, xp, y, yp, z, delta) on the upright phase ellipse of one plane (particles.py:925-983)."""
+    pts = ellipse_edge_points(1 / (amax ** 2), 0.0, 1 / (apmax ** 2), 1.0, number)
+    if plane == "x":
+        return [(p[0], p[1], 0.0, 0.0, 0.0, delta) for p in pts]
+    return [(0.0, 0.0, p[0], p[1], 0.0, delta) for p in pts]
+
+
+# ----------------------------------------------- gantry design search
+def a05_gantry_kwargs(param: Sequence[float], part_per_winding: int = 60) -> Dict:
+    """hust_gantry_section keyword arguments of one 27-value design vector, as
+    final_code/work/optim/A05run.py:26-180 builds HUST_SC_GANTRY from it (no vector: its
+    defaults)."""
+    p = list(param)
+    have = len(p) > 0
+
+    def v(i, d):
+        return p[i] if have else d
+
+    ag1, ag2 = (int(p[12]), int(p[13])) if have else (20, 20)
+    ag3, ag4, ag5 = (int(p[24]), int(p[25]), int(p[26])) if have else (20, 20, 20)
+    ag345 = ag3 + ag4 + ag5
+    return dict(
+        DL1=0.8541973, GAP1=0.45, GAP2=0.45,
+        qs1_length=0.27, qs1_aperture_radius=60 * MM, qs1_gradient=v(0, 0), qs1_second_gradient=v(2, 0),
+        qs2_length=0.27, qs2_aperture_radius=60 * MM, qs2_gradient=v(1, 0), qs2_second_gradient=v(3, 0),
+        cct12_big_r=0.95,
+        agcct12_inner_small_r=92.5 * MM - 20 * MM + 17 * MM, agcct12_outer_small_r=108.5 * MM - 20 * MM + 17 * MM,
+        dicct12_inner_small_r=124.5 * MM - 20 * MM + 17 * MM, dicct12_outer_small_r=140.5 * MM - 20 * MM + 17 * MM,
+        agcct1_winding_number=ag1, agcct2_winding_number=ag2, dicct12_winding_number=42,
+        dicct12_bending_angle=22.5,
+        agcct1_bending_angle=22.5 * (ag1 / (ag1 + ag2)), agcct2_bending_angle=22.5 * (ag2 / (ag1 + ag2)),
+        dicct12_tilt_angles=[30, v(4, 90), v(5, 90), v(6, 90)],
+        agcct12_tilt_angles=[v(7, 90), 30, v(8, 90), v(9, 90)],
+        dicct12_current=v(10, 10000), agcct12_current=v(11, 10000),
+        DL2=2.6, GAP3=0.45,
+        qs3_length=0.27, qs3_aperture_radius=60 * MM, qs3_gradient=v(14, 0), qs3_second_gradient=v(15, 0),
+        cct345_big_r=0.95,
+        agcct345_inner_small_r=92.5 * MM + 0.1 * MM + 17 * MM, agcct345_outer_small_r=108.5 * MM + 0.1 * MM + 17 * MM,
+        dicct345_inner_small_r=124.5 * MM + 0.1 * MM + 17 * MM,
+        dicct345_outer_small_r=140.5 * MM + 0.1 * MM + 17 * MM,
+        agcct3_winding_number=ag3, agcct4_winding_number=ag4, agcct5_winding_number=ag5,
+        dicct345_winding_number=128, dicct345_bending_angle=-67.5,
+        agcct3_bending_angle=-67.5 * (ag3 / ag345), agcct4_bending_angle=-67.5 * (ag4 / ag345),
+        agcct5_bending_angle=-67.5 * (ag5 / ag345),
+        dicct345_tilt_angles=[30, v(16, 90), v(17, 90), v(18, 90)],
+        agcct345_tilt_angles=[v(19, 90), 30, v(20, 90), v(21, 90)],
+        dicct345_current=v(22, 10000), agcct345_current=v(23, 10000),
+        part_per_winding=part_per_winding,
+    )
+
+
+def a05_objectives(pps: Sequence[Tuple], group: int) -> List[float]:
+    """The 11 objectives of one design from its end-plane phase-space particles
+    (final_code/work/optim/A05run.py:248-330): x, y in mm and xp, yp in mrad; absolute maxima,
+    |averages|, |max|x| - max|y||, and the spread (max - min) of the half widths (max - min)/2
+    of x and of y over consecutive groups of `group` particles."""
+    x = [pp[0] / MM for pp in pps]
+    xp = [pp[1] / MRAD for pp in pps]
+    y = [pp[2] / MM for pp in pps]
+    yp = [pp[3] / MRAD for pp in pps]
+    hwx, hwy = [], []
+    for g in range(0, len(pps), group):
+        gx, gy = x[g:g + group], y[g:g + group]
+        hwx.append((numpy.max(gx) - numpy.min(gx)) / 2)
+        hwy.append((numpy.max(gy) - numpy.min(gy)) / 2)
+    amax = [float(numpy.max(numpy.abs(a))) for a in (x, xp, y, yp)]
+    cen = [abs(sum(a) / len(a)) for a in (x, xp, y, yp)]
+    return amax + cen + [abs(amax[0] - amax[2]), float(numpy.max(hwx) - numpy.min(hwx)),
+                         float(numpy.max(hwy) - numpy.min(hwy))]
