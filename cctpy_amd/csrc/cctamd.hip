@@ -168,29 +168,83 @@ __device__ __forceinline__ void tile_terms_f64(const double* __restrict__ seg64,
                                                const double (&py)[NP], const double (&pz)[NP], double (&ax)[NP],
                                                double (&ay)[NP], double (&az)[NP]) {
     const double2* s = reinterpret_cast<const double2*>(seg64) + t * kTile * 4;
-#pragma unroll 4
-    for (int k = 0; k < kTile; ++k) {
-        const double2 k01 = s[4 * k], k2_ = s[4 * k + 1], m01 = s[4 * k + 2], m2_ = s[4 * k + 3];
+    // one point at a time and no unrolling: this path is rare, and its registers count
+    // against the occupancy of the FP32 loop of the same kernel
 #pragma unroll
-        for (int j = 0; j < NP; ++j) {
+    for (int j = 0; j < NP; ++j) {
+        double bx = ax[j], by = ay[j], bz = az[j];
+#pragma unroll 1
+        for (int k = 0; k < kTile; ++k) {
+            const double2 k01 = s[4 * k], k2_ = s[4 * k + 1], m01 = s[4 * k + 2], m2_ = s[4 * k + 3];
             const double rx = px[j] - m01.x, ry = py[j] - m01.y, rz = pz[j] - m2_.x;
             const double r2 = rx * rx + ry * ry + rz * rz;
             const double inv = rsqrt_f64(r2);
             const double inv3 = inv * inv * inv;
-            ax[j] += (k01.y * rz - k2_.x * ry) * inv3;
-            ay[j] += (k2_.x * rx - k01.x * rz) * inv3;
-            az[j] += (k01.x * ry - k01.y * rx) * inv3;
+            bx += (k01.y * rz - k2_.x * ry) * inv3;
+            by += (k2_.x * rx - k01.x * rz) * inv3;
+            bz += (k01.x * ry - k01.y * rx) * inv3;
         }
+        ax[j] = bx;
+        ay[j] = by;
+        az[j] = bz;
     }
 }
 
-// A tile is *near* a point when |p - c|^2 < kNearTile e^2 (c the tile origin, e
-// the tile extent, the largest |m - c| of its segments): within 2.5 extents.
-// Near tiles are summed in FP64 (tile_terms_f64) by every FP32 kernel, for the
-// whole wave (or lane group) when any of its points is near: a beam that leaves
-// the bore touches few tiles, so this costs little (0.2-0.7 % of point-tile
-// pairs on the AG-CCT section's recorded tracks).
+// tile_terms_f64 for a lane group of LG lanes that share the tile (the split-wave
+// kernels): in each pass every lane loads one segment's FP64 row (the loads of a
+// pass go out together, and each row is read once per group), and the group's
+// lanes take the segments in order from one another by lane shuffles.  gl is the
+// lane's index in its group; the group's lanes are consecutive and all active.
+template <int LG, int NP>
+__device__ __forceinline__ void group_tile_terms_f64(const double* __restrict__ seg64, int64_t t, int gl,
+                                                     const double (&px)[NP], const double (&py)[NP],
+                                                     const double (&pz)[NP], double (&ax)[NP], double (&ay)[NP],
+                                                     double (&az)[NP]) {
+    const int first = (int)(threadIdx.x & (kWave - 1)) - gl;  // lane 0 of the group in the wave
+    // one point at a time (the shuffles repeat per point): this path is rare, and its
+    // registers count against the occupancy of the kernel's FP32 loop
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        double bx = ax[j], by = ay[j], bz = az[j];
+#pragma unroll 1
+        for (int k0 = 0; k0 < kTile; k0 += LG) {
+            const double* r = seg64 + (t * kTile + k0 + gl) * 8;
+            const double2 k01 = *reinterpret_cast<const double2*>(r), m01 = *reinterpret_cast<const double2*>(r + 4);
+            const double k2 = r[2], m2 = r[6];
+#pragma unroll 1
+            for (int k = 0; k < LG; ++k) {
+                const int src = first + k;
+                const double kx = __shfl(k01.x, src), ky = __shfl(k01.y, src), kz = __shfl(k2, src);
+                const double rx = px[j] - __shfl(m01.x, src), ry = py[j] - __shfl(m01.y, src);
+                const double rz = pz[j] - __shfl(m2, src);
+                const double r2 = rx * rx + ry * ry + rz * rz;
+                const double inv = rsqrt_f64(r2);
+                const double inv3 = inv * inv * inv;
+                bx += (ky * rz - kz * ry) * inv3;
+                by += (kz * rx - kx * rz) * inv3;
+                bz += (kx * ry - ky * rx) * inv3;
+            }
+        }
+        ax[j] = bx;
+        ay[j] = by;
+        az[j] = bz;
+    }
+}
+
+// Tiles close to a point.  With e the tile extent (largest |m - c| of its
+// segments, c the tile origin) and q = p - c:
+//  * |q| < 1.5 e (kFp64Tile): the point is among the tile's segments; rounding q
+//    or m - c to FP32 moves the nearest segment's term by ~1e-6 of itself, so
+//    the tile is summed in FP64 (tile_terms_f64);
+//  * 1.5 e <= |q| < 2.5 e (kNearTile): the K x q - C form would cancel by up to
+//    ~7x, so the K/C kernels sum the tile in the FP32 direct form (kl x r) |r|^-3;
+//  * farther: the kernel's own form.
+// Each decision is taken for the whole wave (or lane group) when any of its
+// points qualifies.  On the AG-CCT section's recorded tracks the FP64 band holds
+// 0.03 % of point-tile pairs and keeps the field within 3.5e-7 of FP64 at every
+// state (against 3.0e-7 with FP64 for the whole 2.5 e band, 7x the pairs).
 constexpr double kNearTile = 6.25;
+constexpr double kFp64Tile = 2.25;
 
 // FP32 terms on tiles re-centred at an FP64 origin; each tile's 32 terms are
 // summed in FP32 and the tile sums are accumulated in FP64; near tiles in FP64.
@@ -214,13 +268,13 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32(const float* __restrict_
     const int tend = w.tile_start + w.tile_count;
     for (int t = w.tile_start + wave; t < tend; t += kWavesPerWG) {
         const double ox = tile_org[4 * t], oy = tile_org[4 * t + 1], oz = tile_org[4 * t + 2];
-        const double near2 = kNearTile * tile_org[4 * t + 3];
+        const double f64_2 = kFp64Tile * tile_org[4 * t + 3];
         float qx[PPL], qy[PPL], qz[PPL], sx[PPL], sy[PPL], sz[PPL];
         bool near = false;
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             const double dx = px[j] - ox, dy = py[j] - oy, dz = pz[j] - oz;
-            near = near || dx * dx + dy * dy + dz * dz < near2;
+            near = near || dx * dx + dy * dy + dz * dz < f64_2;
             qx[j] = (float)dx;
             qy[j] = (float)dy;
             qz[j] = (float)dz;
@@ -263,7 +317,7 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32(const float* __restrict_
 // |q|^2 is formed in FP64 per tile.  Both rewrites cancel when the point is
 // close to the tile (|q| of the order of the tile extent e): for |q| >= 2.5 e
 // the factors (|q|^2 + e^2) / |r|^2 and |q| / |r| stay below 3.3; nearer tiles
-// are summed in FP64 (see kNearTile).
+// are summed in the direct form or in FP64 (see kNearTile).
 
 // Waves per workgroup of the scalar-load Biot-Savart kernels (K/C FP32 and FP64).
 // Scalar loads return out of order, so a wave waits for all of its segment loads
@@ -274,6 +328,19 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32(const float* __restrict_
 #define CCTAMD_BS_WAVES 8
 #endif
 constexpr int kBsWaves = CCTAMD_BS_WAVES;
+
+// One FP32 direct-form term (kl x r) |r|^-3 from a K/C row, r = q - m.
+__device__ inline void tile_direct_term(const float4& a, const float4& b, const float4& c, float qx, float qy,
+                                        float qz, float& bx, float& by, float& bz) {
+    // m = -0.5 (-2m) is exact in FP32
+    const float rx = __builtin_fmaf(0.5f, b.z, qx), ry = __builtin_fmaf(0.5f, b.w, qy), rz = __builtin_fmaf(0.5f, c.x, qz);
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float w3 = inv * inv * inv;
+    bx += (a.y * rz - a.z * ry) * w3;
+    by += (a.z * rx - a.x * rz) * w3;
+    bz += (a.x * ry - a.y * rx) * w3;
+}
 
 template <int PPL, int NW>
 __global__ __launch_bounds__(kWave * NW) void k_bs_partial_f32k(const float* __restrict__ seg32k,
@@ -295,14 +362,15 @@ __global__ __launch_bounds__(kWave * NW) void k_bs_partial_f32k(const float* __r
     const int tend = w.tile_start + w.tile_count;
     for (int t = w.tile_start + wave; t < tend; t += NW) {
         const double ox = tile_org[4 * t], oy = tile_org[4 * t + 1], oz = tile_org[4 * t + 2];
-        const double near2 = kNearTile * tile_org[4 * t + 3];
+        const double e2 = tile_org[4 * t + 3];
         float qx[PPL], qy[PPL], qz[PPL], qq[PPL], kx[PPL], ky[PPL], kz[PPL], cx[PPL], cy[PPL], cz[PPL];
-        bool near = false;
+        bool near = false, f64 = false;
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             const double dx = px[j] - ox, dy = py[j] - oy, dz = pz[j] - oz;
             const double d2 = dx * dx + dy * dy + dz * dz;
-            near = near || d2 < near2;
+            near = near || d2 < kNearTile * e2;
+            f64 = f64 || d2 < kFp64Tile * e2;
             qx[j] = (float)dx;
             qy[j] = (float)dy;
             qz[j] = (float)dz;
@@ -310,8 +378,23 @@ __global__ __launch_bounds__(kWave * NW) void k_bs_partial_f32k(const float* __r
             kx[j] = ky[j] = kz[j] = cx[j] = cy[j] = cz[j] = 0.0f;
         }
         const float4* s = reinterpret_cast<const float4*>(seg32k) + (int64_t)t * kTile * 3;
-        if (__any(near)) {  // wave-uniform: FP64 terms for the whole wave on this tile
+        if (__any(f64)) {  // wave-uniform: FP64 terms for the whole wave on this tile
             tile_terms_f64<PPL>(seg64, t, px, py, pz, ax, ay, az);
+            continue;
+        }
+        if (__any(near)) {  // wave-uniform: FP32 direct form
+#pragma unroll 1
+            for (int k = 0; k < kTile; ++k) {
+                const float4 a = s[3 * k], b = s[3 * k + 1], c = s[3 * k + 2];
+#pragma unroll
+                for (int j = 0; j < PPL; ++j) tile_direct_term(a, b, c, qx[j], qy[j], qz[j], kx[j], ky[j], kz[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                ax[j] += (double)kx[j];
+                ay[j] += (double)ky[j];
+                az[j] += (double)kz[j];
+            }
             continue;
         }
 #pragma unroll 8
@@ -400,6 +483,18 @@ __device__ __forceinline__ void kc_row_steps(const KCRow& g, const float (&qx)[2
     if constexpr (S + 1 < 16) kc_row_steps<S + 1>(g, qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
 }
 
+template <int S>
+__device__ __forceinline__ void direct_row_steps(const KCRow& g, const float (&qx)[2], const float (&qy)[2],
+                                                 const float (&qz)[2], float (&bx)[2], float (&by)[2],
+                                                 float (&bz)[2]) {
+    const KCRow h{make_float4(row_bcast<S>(g.a.x), row_bcast<S>(g.a.y), row_bcast<S>(g.a.z), 0.0f),
+                  make_float4(0.0f, 0.0f, row_bcast<S>(g.b.z), row_bcast<S>(g.b.w)),
+                  make_float4(row_bcast<S>(g.c.x), 0.0f, 0.0f, 0.0f)};
+#pragma unroll
+    for (int j = 0; j < 2; ++j) tile_direct_term(h.a, h.b, h.c, qx[j], qy[j], qz[j], bx[j], by[j], bz[j]);
+    if constexpr (S + 1 < 16) direct_row_steps<S + 1>(g, qx, qy, qz, bx, by, bz);
+}
+
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(6))) void k_bs_partial_f32k_row(const float* __restrict__ seg32k,
                                                              const double* __restrict__ tile_org,
                                                              const double* __restrict__ seg64,
@@ -410,12 +505,16 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(6))) void k
     constexpr int PB = 2 * LG;              // 32 points per block
     constexpr int NS = kWavesPerWG * 4;     // tile streams (rows) per workgroup
     __shared__ double red[kWavesPerWG][3][PB];
+    // each lane's FP64 sums live in LDS (its own slots, no synchronisation): updated once
+    // per tile, they would otherwise hold 12 registers through the FP32 loop
+    __shared__ double acc[6][kWG];  // [component * 2 + point][thread]
     const Work w = work[blockIdx.x];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x >> 6;
     const int gl = lane & (LG - 1);
     const int stream = wave * 4 + (lane >> 4);
-    double px[2], py[2], pz[2], ax[2], ay[2], az[2];
+    const int tid = threadIdx.x;
+    double px[2], py[2], pz[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         int idx = gl + LG * j;
@@ -424,8 +523,9 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(6))) void k
         px[j] = pts[gp];
         py[j] = pts[ptot + gp];
         pz[j] = pts[2 * ptot + gp];
-        ax[j] = ay[j] = az[j] = 0.0;
     }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c][tid] = 0.0;
     const int tend = w.tile_start + w.tile_count;
     const float4* src = reinterpret_cast<const float4*>(seg32k);
     auto load = [&](int t, KCRow& g0, KCRow& g1) {  // segments gl and gl + 16 of tile t
@@ -440,34 +540,53 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(6))) void k
         KCRow g0, g1;
         load(t, g0, g1);
         const double ox = tile_org[4 * t], oy = tile_org[4 * t + 1], oz = tile_org[4 * t + 2];
-        const double near2 = kNearTile * tile_org[4 * t + 3];
+        const double e2 = tile_org[4 * t + 3];
         float qx[2], qy[2], qz[2], qq[2], kx[2], ky[2], kz[2], cx[2], cy[2], cz[2];
-        bool near = false;
+        bool near = false, f64 = false;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const double dx = px[j] - ox, dy = py[j] - oy, dz = pz[j] - oz;
             const double d2 = dx * dx + dy * dy + dz * dz;
-            near = near || d2 < near2;
+            near = near || d2 < kNearTile * e2;
+            f64 = f64 || d2 < kFp64Tile * e2;
             qx[j] = (float)dx;
             qy[j] = (float)dy;
             qz[j] = (float)dz;
             qq[j] = (float)d2;
             kx[j] = ky[j] = kz[j] = cx[j] = cy[j] = cz[j] = 0.0f;
         }
-        if (__any(near)) {  // FP64 terms for the active rows on this round, each row its own tile
-            tile_terms_f64<2>(seg64, t, px, py, pz, ax, ay, az);
+        if (__any(f64)) {  // FP64 terms for the active rows on this round, each row its own tile
+            double ax[2] = {acc[0][tid], acc[1][tid]}, ay[2] = {acc[2][tid], acc[3][tid]};
+            double az[2] = {acc[4][tid], acc[5][tid]};
+            group_tile_terms_f64<LG, 2>(seg64, t, gl, px, py, pz, ax, ay, az);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                acc[j][tid] = ax[j];
+                acc[2 + j][tid] = ay[j];
+                acc[4 + j][tid] = az[j];
+            }
+        } else if (__any(near)) {  // FP32 direct form for the active rows on this round
+            direct_row_steps<0>(g0, qx, qy, qz, kx, ky, kz);
+            direct_row_steps<0>(g1, qx, qy, qz, kx, ky, kz);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                acc[j][tid] += (double)kx[j];
+                acc[2 + j][tid] += (double)ky[j];
+                acc[4 + j][tid] += (double)kz[j];
+            }
         } else {
             kc_row_steps<0>(g0, qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
             kc_row_steps<0>(g1, qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                ax[j] += (double)((ky[j] * qz[j] - kz[j] * qy[j]) - cx[j]);
-                ay[j] += (double)((kz[j] * qx[j] - kx[j] * qz[j]) - cy[j]);
-                az[j] += (double)((kx[j] * qy[j] - ky[j] * qx[j]) - cz[j]);
+                acc[j][tid] += (double)((ky[j] * qz[j] - kz[j] * qy[j]) - cx[j]);
+                acc[2 + j][tid] += (double)((kz[j] * qx[j] - kx[j] * qz[j]) - cy[j]);
+                acc[4 + j][tid] += (double)((kx[j] * qy[j] - ky[j] * qx[j]) - cz[j]);
             }
         }
         t = tn;
     }
+    double ax[2] = {acc[0][tid], acc[1][tid]}, ay[2] = {acc[2][tid], acc[3][tid]}, az[2] = {acc[4][tid], acc[5][tid]};
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
 #pragma unroll
@@ -544,14 +663,15 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __rest
         const bool real = t0 + stream < tend;
         const int t = real ? t0 + stream : w.tile_start;
         const double ox = tile_org[4 * t], oy = tile_org[4 * t + 1], oz = tile_org[4 * t + 2];
-        const double near2 = kNearTile * tile_org[4 * t + 3];
+        const double e2 = tile_org[4 * t + 3];
         float qx[PL], qy[PL], qz[PL], qq[PL], kx[PL], ky[PL], kz[PL], cx[PL], cy[PL], cz[PL];
-        bool near = false;
+        bool near = false, f64 = false;
 #pragma unroll
         for (int j = 0; j < PL; ++j) {
             const double dx = px[j] - ox, dy = py[j] - oy, dz = pz[j] - oz;
             const double d2 = dx * dx + dy * dy + dz * dz;
-            near = near || (real && d2 < near2);
+            near = near || (real && d2 < kNearTile * e2);
+            f64 = f64 || (real && d2 < kFp64Tile * e2);
             qx[j] = (float)dx;
             qy[j] = (float)dy;
             qz[j] = (float)dz;
@@ -559,8 +679,24 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __rest
             kx[j] = ky[j] = kz[j] = cx[j] = cy[j] = cz[j] = 0.0f;
         }
         const float4* r4 = lds[stream];
-        if (__any(near)) {  // FP64 terms for this round, each lane group its own tile (if any)
-            if (real) tile_terms_f64<PL>(seg64, t, px, py, pz, ax, ay, az);
+        if (__any(f64)) {  // FP64 terms for this round, each lane group its own tile (if any)
+            if (real) group_tile_terms_f64<LG, PL>(seg64, t, gl, px, py, pz, ax, ay, az);
+            __syncthreads();
+            continue;
+        }
+        if (__any(near)) {  // FP32 direct form for this round
+#pragma unroll 1
+            for (int k = 0; k < kTile; ++k) {
+                const float4 a = r4[3 * k], b = r4[3 * k + 1], c = r4[3 * k + 2];
+#pragma unroll
+                for (int j = 0; j < PL; ++j) tile_direct_term(a, b, c, qx[j], qy[j], qz[j], kx[j], ky[j], kz[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < PL; ++j) {
+                ax[j] += (double)kx[j];
+                ay[j] += (double)ky[j];
+                az[j] += (double)kz[j];
+            }
             __syncthreads();
             continue;
         }
