@@ -60,14 +60,18 @@ def kernel_name(ppl: int) -> str:
 
 
 def measured_traffic(workload: str, kernel: str, nwork: int):
-    """HBM bytes per launch of this kernel shape from the committed counter summary, or None."""
+    """HBM bytes per launch of this kernel shape from the committed counter summary, or None.  Entries
+    carry the kernel-set version they were measured with (autotune.KERNEL_SET, raised with every
+    kernel change): a summary of older kernels gives None."""
+    from cctpy_amd import autotune
     try:
         with open(TRAFFIC_FILE) as f:
             rows = json.load(f)["launches"]
     except (OSError, ValueError, KeyError):
         return None
     for r in rows:
-        if r["workload"] == workload and r["kernel"] == kernel and r["work_items"] == nwork:
+        if (r.get("kernel_set") == autotune.KERNEL_SET and r["workload"] == workload and r["kernel"] == kernel
+                and r["work_items"] == nwork):
             return r["hbm_bytes_per_launch"]
     return None
 
@@ -85,38 +89,45 @@ def copy_calibration(mib: int, reps: int = 5) -> int:
     return 0
 
 
-def cpu_baseline(wl, n_particles: int, n_steps: int):
-    """C restatement of the reference CPU path (FP64, OpenMP over particles) on a bounded sample."""
-    import numpy as np
-    from oracle import c_oracle, cct_oracle as O, workload_builder as WB
+def cpu_baseline(workload: str, sample: str, workers: int = 16):
+    """The reference CPU path as the oracle restates it, on a bounded sample of the workload, in a
+    child process that never touches the GPU (oracle/cpu_baseline.py): the numpy restatement
+    (per-particle RK4, numpy Biot-Savart per CCT, the reference's operation order) in one process
+    and in a pool of `workers` processes (one task per proton, like
+    ParticleRunner.run_only(concurrency_level=os.cpu_count())), and the C restatement (OpenMP
+    over protons) on the pool's sample.  sample = "N1xS1,NPxSP": protons x RK4 steps of the
+    one-process and of the pool runs.  Called before this process touches the GPU."""
+    import subprocess
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    threads = max(1, min(16, cores))
-    bl = WB.build_beamline(wl.designs[0])
-    mags = O.beamline_arrays(bl.magnets)
-    b = wl.beams[0]
-    st, kq, speed = WB.beam_state(b.coords[:n_particles], b.s0, b.kinetic_mev, bl)
-    dts = np.array([O.rk4_steps(wl.beam_length(0), wl.footstep, s)[1] for s in speed])
-    ns = np.full(len(st), n_steps, np.int64)
-    c_oracle.track(mags, st[:2], kq[:2], dts[:2], np.ones(2, np.int64), threads=threads)  # warm
-    t0 = time.perf_counter()
-    c_oracle.track(mags, st, kq, dts, ns, threads=threads)
-    el = time.perf_counter() - t0
-    n1 = max(1, n_particles // threads)  # one core: a proportional slice of the same sample
-    t0 = time.perf_counter()
-    c_oracle.track(mags, st[:n1], kq[:n1], dts[:n1], ns[:n1], threads=1)
-    el1 = time.perf_counter() - t0
-    nseg = int(mags["kl"].shape[0])
-    psteps = len(st) * n_steps
+    workers = max(1, min(workers, cores))
+    one, pool = sample.split(",")
+    n1, s1 = (int(v) for v in one.split("x"))
+    np_, sp = (int(v) for v in pool.split("x"))
+    res = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", workload, str(n1), str(s1), str(np_), str(sp),
+                          str(workers)], cwd=ROOT, capture_output=True, text=True, timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError("cpu baseline failed: " + res.stderr[-2000:])
+    r = json.loads(res.stdout.strip().splitlines()[-1])
     model = open("/proc/cpuinfo").read().split("model name")[1].split("\n")[0].strip(" :\t") \
         if os.path.exists("/proc/cpuinfo") else "unknown"
-    return {"value": psteps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{len(st)} protons x {n_steps} RK4 steps over all {nseg} segments + QS of {wl.name} "
-                      f"(C FP64 restatement of the reference CPU path, OpenMP {threads} threads, {el:.1f} s, "
-                      f"host cores visible {cores}, CPU {model})",
-            "pair_evals_per_s": psteps * 4 * nseg / el, "value_1core": n1 * n_steps / el1}
+    rate = r["numpy_pool"]["particle_steps_per_s"]
+    pairs = 4 * r["segments"]
+    return {"value": rate, "unit": "particle-steps/s", "cores": workers, "kind": "port",
+            "sample": f"{np_} protons x {sp} RK4 steps of {workload} over all {r['segments']} segments + QS: numpy "
+                      f"restatement of the reference CPU path (per-particle RK4, numpy Biot-Savart per CCT), "
+                      f"{workers} processes, one task per proton ({r['numpy_pool']['seconds']:.1f} s); host cores "
+                      f"visible {cores}, CPU {model}",
+            "pair_evals_per_s": rate * pairs,
+            "numpy_1_process": {"particle_steps_per_s": r["numpy_1proc"]["particle_steps_per_s"],
+                                "pair_evals_per_s": r["numpy_1proc"]["particle_steps_per_s"] * pairs,
+                                "sample": f"{n1} protons x {s1} RK4 steps, one process"},
+            "c_openmp": {"particle_steps_per_s": r["c_openmp"]["particle_steps_per_s"],
+                         "pair_evals_per_s": r["c_openmp"]["particle_steps_per_s"] * pairs,
+                         "threads": workers, "kind": "port",
+                         "sample": f"C FP64 restatement, OpenMP {workers} threads, same {np_} x {sp} sample"}}
 
 
 class Setup:
@@ -257,6 +268,39 @@ class Setup:
                 "grad_finite": bool(torch.isfinite(tr.tgrad).all().item())}
 
 
+def tracking_leg(name: str, max_steps: int = 0, passes: int = 3, ppl: int = 0, items_per_cu: int = 0,
+                 adjoint_steps: int = 0) -> dict:
+    """FP32 and plain FP64 tracking passes of one workload (HIP-graph replay, inputs resident),
+    their final-position difference, and optionally FP32 forward + adjoint over a step prefix.
+    ppl = 0 lets the autotuner pick the block shape (it tracks a sample pass first)."""
+    import numpy as np
+    from cctpy_amd import workloads as W
+    S = Setup(W.by_name(name), align_cct=adjoint_steps > 0, max_steps=max_steps)
+    sh32, sh64 = S.shape(FP32, ppl, items_per_cu), S.shape(64, ppl, items_per_cu)
+    ms32, out32, plan32, _ = S.forward(FP32, sh32, passes, 1, True)
+    ms64, out64, _, _ = S.forward(64, sh64, max(1, passes // 2), 1, True)
+    d = np.linalg.norm(out32[:, :3] - out64[:, :3], axis=1)
+    res = {"segments": int(S.model.nseg), "protons": S.P, "rk4_steps": S.max_steps, "steps_capped": max_steps > 0,
+           "footstep_mm": S.wl.footstep * 1e3, "hip_graph": True,
+           "fp32": {"ms_per_pass": ms32, "particle_steps_per_s": S.particle_steps / (ms32 / 1e3),
+                    "pair_evals_per_s": S.pairs_per_pass / (ms32 / 1e3), "shape": sh32},
+           "fp64": {"ms_per_pass": ms64, "particle_steps_per_s": S.particle_steps / (ms64 / 1e3),
+                    "pair_evals_per_s": S.pairs_per_pass / (ms64 / 1e3), "shape": sh64},
+           "fp32_vs_fp64": ms64 / ms32,
+           "launches_per_rk4_step": 8,
+           "position_error_um_vs_fp64": {"max": float(d.max() * 1e6), "median": float(np.median(d) * 1e6),
+                                         "p99": float(np.percentile(d, 99) * 1e6),
+                                         "within_1um": float(np.mean(d <= 1e-6))}}
+    if adjoint_steps > 0:
+        A_ = Setup(W.by_name(name), align_cct=True, max_steps=adjoint_steps)
+        res["forward_adjoint"] = dict(rk4_steps=A_.max_steps, **A_.forward_adjoint(
+            A_.shape(FP32, 2, 4), 1, 0, ckpt_every=0, tune=False))
+        del A_
+    log(f"[bench] {name}: fp32 {ms32:.1f} ms, fp64 {ms64:.1f} ms per pass of {S.max_steps} steps, error max "
+        f"{d.max() * 1e6:.3g} um")
+    return res
+
+
 def field_along_arc(n_points: int = 1000, reps: int = 20) -> dict:
     """Field of the single-CCT workload (23,040 segments) at points spaced evenly along the ideal
     arc in the median plane: FP64 and FP32 paths timed (HIP events, inputs on the device), the
@@ -351,7 +395,8 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison pass")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip the population and adjoint extras")
-    ap.add_argument("--cpu-sample", default="64x24", help="CPU baseline sample: particles x RK4 steps")
+    ap.add_argument("--cpu-sample", default="4x12,32x12",
+                    help="CPU baseline samples: protons x RK4 steps of the one-process and of the pool runs")
     ap.add_argument("--roofline-launches", type=int, default=20)
     ap.add_argument("--roofline-precision", type=int, choices=(FP32, 64), default=FP32,
                     help="with --roofline-only: launch the FP32 (33) or the plain FP64 (64) kernel")
@@ -368,6 +413,14 @@ def main() -> int:
         return 2
     if args.copy_mib:
         return copy_calibration(args.copy_mib)
+
+    # the CPU baseline first, in a child process, before this process touches the GPU
+    cpu = None
+    if not (args.no_cpu or args.roofline_only or args.mode == "adjoint"):
+        cpu = cpu_baseline(args.workload, args.cpu_sample)
+        log(f"[bench] cpu baseline: numpy {cpu['value']:.4g} particle-steps/s ({cpu['cores']} processes), "
+            f"{cpu['numpy_1_process']['particle_steps_per_s']:.3g} in one process; C/OpenMP "
+            f"{cpu['c_openmp']['particle_steps_per_s']:.4g}")
 
     import numpy as np
     import torch
@@ -404,7 +457,9 @@ def main() -> int:
         plan = model.plan(S.npt, tuned[FP32]["ppl"], tuned[FP32]["target_items"])
         _, partial = model.workspace(plan)
         avg = bs_launch_timing(plan, S.s0[:3].contiguous(), partial, args.roofline_launches, args.roofline_precision)
-        kname = kernel_name(plan.ppl) if args.roofline_precision == FP32 else f"k_bs_partial_f64<{plan.ppl}, {BS_WAVES}>"
+        kname = kernel_name(plan.ppl) if args.roofline_precision == FP32 else (
+            f"k_bs_partial_f64<{plan.ppl}, {BS_WAVES}>" if plan.ppl > 0 else
+            ("k_bs_partial_f64_grp<4, 2, 2>" if plan.ppl == -2 else "k_bs_partial_f64_grp<8, 2, 4>"))
         print(json.dumps({"workload": wl.name, "kernel": kname, "work_items": plan.nwork,
                           "launches": args.roofline_launches + 1, "avg_launch_ms": avg}), flush=True)
         return 0
@@ -501,6 +556,9 @@ def main() -> int:
         fa0 = A_.forward_adjoint(sh, 1, 1, ckpt_every=0)
         A_P, A_seg, A_steps = A_.P, int(A_.model.nseg), A_.max_steps
         del A_
+        extras["single_cct_tracking"] = tracking_leg("single_cct", passes=5)
+        extras["hust_total"] = tracking_leg("hust_total", max_steps=2000, passes=1, ppl=2, items_per_cu=4,
+                                            adjoint_steps=400)
         extras["single_cct_field_along_arc"] = field_along_arc()
         log(f"[bench] single_cct field at 1000 arc points: fp64 {extras['single_cct_field_along_arc']['fp64']['ms']:.3f} ms, "
             f"fp32 {extras['single_cct_field_along_arc']['fp32']['ms']:.3f} ms")
@@ -510,11 +568,6 @@ def main() -> int:
         log(f"[bench] adjoint_section: fwd+adj {fa['ms_per_pass']:.1f} ms ({fa['forward_ms']:.1f} forward); "
             f"gradient vs FD max rel {gc['max_rel_err_vs_fd']:.3g}")
 
-    cpu = None
-    if not args.no_cpu:
-        n_p, n_s = (int(v) for v in args.cpu_sample.split("x"))
-        cpu = cpu_baseline(wl, n_p, n_s)
-        log(f"[bench] cpu baseline: {cpu['value']:.4g} particle-steps/s ({cpu['cores']} threads)")
 
     out = {
         "metric": METRIC,

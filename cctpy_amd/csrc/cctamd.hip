@@ -157,7 +157,7 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
 
 // FP64 pair terms of tile t (seg64 rows) for NP points, added to FP64 accumulators:
 // the term and summation order of k_bs_partial_f64.  The FP32 kernels take this
-// form for a tile within 2.5 tile extents of one of their points: there the
+// form for a tile that one of their points lies among (kFp64Tile below): there the
 // offset q = p - c from the tile origin is comparable with the distance to the
 // nearest segment, and rounding q (or m - c) to FP32 moves that segment's term by
 // ~1e-6 of itself, more than the whole FP32 sum may differ from the FP64 one.
@@ -233,18 +233,21 @@ __device__ __forceinline__ void group_tile_terms_f64(const double* __restrict__ 
 
 // Tiles close to a point.  With e the tile extent (largest |m - c| of its
 // segments, c the tile origin) and q = p - c:
-//  * |q| < 1.5 e (kFp64Tile): the point is among the tile's segments; rounding q
-//    or m - c to FP32 moves the nearest segment's term by ~1e-6 of itself, so
-//    the tile is summed in FP64 (tile_terms_f64);
-//  * 1.5 e <= |q| < 2.5 e (kNearTile): the K x q - C form would cancel by up to
+//  * |q|^2 < 1.5 e^2 (kFp64Tile, |q| < 1.22 e): the point is among the tile's
+//    segments; rounding q or m - c to FP32 moves the nearest segment's term by
+//    ~1e-6 of itself, so the tile is summed in FP64 (tile_terms_f64);
+//  * up to |q| < 2.5 e (kNearTile): the K x q - C form would cancel by up to
 //    ~7x, so the K/C kernels sum the tile in the FP32 direct form (kl x r) |r|^-3;
 //  * farther: the kernel's own form.
 // Each decision is taken for the whole wave (or lane group) when any of its
-// points qualifies.  On the AG-CCT section's recorded tracks the FP64 band holds
-// 0.03 % of point-tile pairs and keeps the field within 3.5e-7 of FP64 at every
-// state (against 3.0e-7 with FP64 for the whole 2.5 e band, 7x the pairs).
+// points qualifies.  A CPU emulation of the FP32 sums at the 300 worst of 129,024
+// states recorded along the AG-CCT section's track gives the same largest field
+// error (3.5e-7) for FP64 bands from |q|^2 < 1.25 e^2 up to 6.25 e^2, and 8.5e-7 at
+// e^2; tiles of coarse, strongly tilted windings (360 segments per winding) reach
+// 5 cm, which puts the in-bore beam of the single-CCT and population workloads at
+// 1.4-1.6 e from the nearest tile origins, outside this band.
 constexpr double kNearTile = 6.25;
-constexpr double kFp64Tile = 2.25;
+constexpr double kFp64Tile = 1.5;
 
 // FP32 terms on tiles re-centred at an FP64 origin; each tile's 32 terms are
 // summed in FP32 and the tile sums are accumulated in FP64; near tiles in FP64.

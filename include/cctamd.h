@@ -17,9 +17,9 @@
  *              tile's current-carrying midpoints) and extent max |m-c|^2
  *   seg_near   the seg64 rows of the same set, passed beside the FP32 tiles: the FP32
  *              paths (precisions 32 and 33) sum a tile in FP64 from these rows when a
- *              point lies within 2.5 tile extents of its origin (|p-c|^2 < 6.25 e2),
- *              where FP32 terms would lose ~1e-6 of the nearest segment's share;
- *              ignored (may be NULL) at precision 64
+ *              point lies among its segments (|p-c|^2 < 1.5 e2), where FP32 terms would
+ *              lose ~1e-6 of the nearest segment's share; ignored (may be NULL) at
+ *              precision 64
  *   qs         double[Q][16] {O, XI, YI, ZI, length, gradient, second_gradient,
  *              aperture_radius} (the reference's QS record)
  *   bconst     double[D][4]  uniform field per design (sum of uniform magnets)
