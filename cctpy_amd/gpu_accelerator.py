@@ -92,7 +92,17 @@ class GPU_ACCELERATOR:
         # FP32 tracking uses the K x q - C tile sums (precision 33 of the C ABI)
         return 33 if self.precision == 32 else 64
 
+    @staticmethod
+    def _device_only(bls) -> None:
+        """The reference facade tracks CCTs and QS only and raises ValueError for any other magnet
+        (gpu_accelerator.py:1970); ParticleRunner is the path for user magnets."""
+        for bl in bls:
+            parts = device_parts(bl)
+            if parts.foreign:
+                raise ValueError(f"magnet {parts.foreign[0]} cannot be accelerated on the GPU")
+
     def _track_inplace(self, bl: Magnet, ps: List[RunningParticle], distance: float, footstep: float) -> None:
+        self._device_only([bl])
         out, _, _, _ = track_particles(ps, [bl], [0] * len(ps), distance, footstep, precision=self._track_precision)
         for q, row in zip(ps, out):
             q.position = P3(*row[:3])
@@ -125,6 +135,7 @@ class GPU_ACCELERATOR:
             footstep: float) -> List[List[RunningParticle]]:
         """Every particle through every beamline in one batched launch; returns new particles
         [beamline][particle], inputs untouched (gpu_accelerator.py:1912-2044)."""
+        self._device_only(bls)
         allp = [q for _ in bls for q in ps]
         design_of = [d for d in range(len(bls)) for _ in ps]
         out, _, _, _ = track_particles(allp, list(bls), design_of, distance, footstep,

@@ -159,16 +159,17 @@ def track_particles(ps: Sequence[RunningParticle], magnets: Sequence[Magnet], de
     """Track particles (each through the design index design_of[i]) on the GPU.
 
     Returns (state (n, 6), record or None, per-particle step counts and dts),
-    without mutating the inputs.  Designs with a magnet kind the device does
-    not know raise ValueError.  ``ppl`` (points-per-lane code) defaults to the
-    block shape that fits the largest design's particle count.
+    without mutating the inputs.  Designs holding magnets the device does not
+    know (user subclasses of Magnet) are tracked with those magnets' fields
+    added on the host at every RK4 stage (``_track_with_host_magnets``).
+    ``ppl`` (points-per-lane code) defaults to the block shape that fits the
+    largest design's particle count.
     """
     from . import engine
-    specs = []
+    specs, foreign = [], []
     for m in magnets:
         parts = device_parts(m)
-        if parts.foreign:
-            raise ValueError(f"magnet {parts.foreign[0]} cannot be tracked on the GPU")
+        foreign.append(parts.foreign)
         specs.append(parts.spec())
     order = sorted(range(len(ps)), key=lambda i: design_of[i])
     npt = numpy.bincount(numpy.asarray(design_of, numpy.int64), minlength=len(magnets))
@@ -185,13 +186,64 @@ def track_particles(ps: Sequence[RunningParticle], magnets: Sequence[Magnet], de
     model = engine.model_for(specs, fp32=(precision != 64))
     if ppl is None:
         ppl = engine.block_shape_for(npt, precision)
-    out, rec = model.track(state, kq, dt, ns, npt, precision=precision, ppl=ppl, record=record)
+    if any(foreign):
+        design_rows = numpy.repeat(numpy.arange(len(magnets)), npt)
+        out, rec = _track_with_host_magnets(model, [foreign[d] for d in design_rows], state, kq, dt, ns, npt,
+                                            precision, ppl, record)
+    else:
+        out, rec = model.track(state, kq, dt, ns, npt, precision=precision, ppl=ppl, record=record)
     inv = numpy.empty(len(ps), numpy.int64)
     inv[numpy.asarray(order, numpy.int64)] = numpy.arange(len(ps))
     out = out[inv]
     if rec is not None:
         rec = rec[:, inv, :]
     return out, rec, ns[inv], dt[inv]
+
+
+def _track_with_host_magnets(model, foreign_of, state, kq, dt, ns, npt, precision, ppl, record):
+    """RK4 tracking through designs that hold magnets the device does not know (user subclasses of
+    Magnet, whose field only their own ``magnetic_field_at`` gives): at every RK4 stage the device
+    sums the CCT, QS and uniform fields at all particles' stage points in one launch, each
+    particle's foreign magnets add theirs through their Python method (the reference's callback
+    contract, particles.py:514-537), and the stage combination follows the reference's RK4
+    (base_utils.py:303-309) in the device stage kernel's operation order.  foreign_of[i] lists
+    the foreign magnets of particle i's design."""
+    P = state.shape[0]
+    p0, v0 = state[:, :3].copy(), state[:, 3:].copy()
+    max_steps = int(ns.max()) if P else 0
+    rec = numpy.zeros((max_steps + 1, P, 6)) if record else None
+    if record:
+        rec[0] = state
+    k, h = kq[:, None], dt[:, None]
+    h2, h6 = h / 2, h / 6
+
+    def accel(p, v):
+        b = model.field(p, npt, precision=precision, ppl=ppl)
+        for i, mags in enumerate(foreign_of):
+            for m in mags:
+                f = m.magnetic_field_at(P3(*p[i]))
+                b[i] = b[i] + numpy.array([f.x, f.y, f.z])
+        cr = numpy.stack([v[:, 1] * b[:, 2] - v[:, 2] * b[:, 1], -v[:, 0] * b[:, 2] + v[:, 2] * b[:, 0],
+                          v[:, 0] * b[:, 1] - v[:, 1] * b[:, 0]], axis=1)
+        return cr * k
+
+    for step in range(max_steps):
+        live = (step < ns)[:, None]
+        a = accel(p0, v0)
+        kp, kv = v0, a
+        p1, v1 = p0 + v0 * h2, v0 + a * h2
+        a = accel(p1, v1)
+        kp, kv = kp + v1 * 2, kv + a * 2
+        p2, v2 = p0 + v1 * h2, v0 + a * h2
+        a = accel(p2, v2)
+        kp, kv = kp + v2 * 2, kv + a * 2
+        p3, v3 = p0 + v2 * h, v0 + a * h
+        a = accel(p3, v3)
+        p0 = numpy.where(live, p0 + (kp + v3) * h6, p0)
+        v0 = numpy.where(live, v0 + (kv + a) * h6, v0)
+        if record:
+            rec[step + 1] = numpy.where(live, numpy.concatenate([p0, v0], axis=1), rec[step + 1])
+    return numpy.concatenate([p0, v0], axis=1), rec
 
 
 class ParticleRunner:

@@ -345,8 +345,52 @@ def test_edge_cases():
     assert rel(p.position.to_list(), op.position) < 1e-13
     b = zero.magnetic_field_at(P3(0.5, 0.0, 0.0))
     assert b.to_list() == [0.0, 0.0, 0.0]
+    # the reference GPU facade refuses magnets it does not know (gpu_accelerator.py:1970)
     with pytest.raises(ValueError):
         class Foreign(Magnet):
             def magnetic_field_at(self, point):
                 return P3()
-        ParticleRunner.run_only(q, Foreign(), 0.1)
+        GPU_ACCELERATOR().track_one_particle_with_multi_qs(Foreign(), q, 0.1, 0.01)
+
+
+def test_run_only_through_a_user_magnet():
+    """ParticleRunner.run_only tracks through any Magnet subclass (the reference's callback
+    contract, particles.py:514-537): a beamline whose uniform field comes from a user class
+    tracks like the same beamline with the device's own UniformMagnet, and a user-magnet-only
+    design matches the oracle's RK4 in that field."""
+    from cctpy_amd.magnets import UniformMagnet
+
+    class UserUniform(Magnet):
+        def __init__(self, b):
+            self.b = b
+
+        def magnetic_field_at(self, point):
+            return self.b.copy()
+
+    b0 = P3(0.0, 0.0, 0.05)
+    rng = np.random.default_rng(19)
+
+    def section(extra):
+        bl = HUST_SC_GANTRY(part_per_winding=60).create_second_bending_part_beamline()
+        bl.magnets.append(extra)
+        return bl
+    bl_user, bl_dev = section(UserUniform(b0)), section(UniformMagnet(b0))
+    ip = ParticleFactory.create_proton_along(bl_user, 2.0, 250)
+    cs = ip.get_natural_coordinate_system()
+    from cctpy_amd.particles import PhaseSpaceParticle
+    ps = [ParticleFactory.create_from_phase_space_particle(
+        ip, cs, PhaseSpaceParticle(*rng.normal(0, [1e-3, 1e-3, 1e-3, 1e-3]), 0, rng.uniform(-0.02, 0.02)))
+        for _ in range(5)]
+    got, rec, ns, _ = track_particles(ps, [bl_user], [0] * 5, 0.05, 2e-3, record=True)
+    want, rec2, _, _ = track_particles(ps, [bl_dev], [0] * 5, 0.05, 2e-3, record=True)
+    assert np.max(np.abs(got[:, :3] - want[:, :3])) < 1e-13
+    for i in range(5):  # rows past a particle's own step count carry nothing
+        assert np.max(np.abs(rec[:ns[i] + 1, i, :3] - rec2[:ns[i] + 1, i, :3])) < 1e-13
+    # a design with only a user magnet: the oracle's RK4 in the same uniform field
+    q = ParticleFactory.create_proton(P3(), P3(1, 0, 0), 250)
+    q0 = q.copy()
+    ParticleRunner.run_only(q, UserUniform(P3(0, 0, 2.4321282996)), np.pi / 2, 20 * MM)
+    op = O.create_proton((0, 0, 0), (1, 0, 0), 250)
+    O.run_only(op, lambda p: (0.0, 0.0, 2.4321282996), np.pi / 2, 20 * MM)
+    assert rel(q.position.to_list(), op.position) < 1e-15 and rel(q.velocity.to_list(), op.velocity) < 1e-15
+    assert q.distance == q0.distance + np.pi / 2
