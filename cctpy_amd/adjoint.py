@@ -38,6 +38,8 @@ class AdjointTracker:
         ``items_per_cu`` its work items per CU (None: the engine's default)."""
         if ckpt_every < 0:
             raise ValueError("ckpt_every must be >= 0")
+        if ppl not in (1, 2, 4, -2, -4):
+            raise ValueError(f"points-per-lane code {ppl} is not one of 1, 2, 4, -2, -4")
         specs = []
         for m in magnets:
             parts = device_parts(m)
@@ -89,6 +91,9 @@ class AdjointTracker:
         kernel dominates the adjoint and prefers other block shapes than the forward alone (2
         points per lane against 1 on a 256-proton section), so the shape is chosen on both.  The
         tracker is left prepared for ``nsteps`` with the chosen shape."""
+        bad = [c for c in candidates if c not in (1, 2, 4, -2, -4)]
+        if bad:
+            raise ValueError(f"points-per-lane codes {bad} are not among 1, 2, 4, -2, -4")
         ns = torch.clamp(nsteps, max=probe_steps)
         best, best_ms = (self.ppl, self.items_per_cu), math.inf
         for ppl, ipc in [(a, b) for a in candidates for b in items_per_cu]:

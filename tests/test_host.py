@@ -250,3 +250,14 @@ def test_design_parameter_map_follows_the_device_order_and_guards_tilts():
     assert MAX_GRAD_TILTS == 4
     with pytest.raises(ValueError, match="tilt angle 5"):
         prob.parameter_map(np.array([5.0, 80.0, 60.0]))
+
+
+def test_adjoint_tracker_refuses_unknown_block_shapes_before_any_device_work():
+    """Points-per-lane codes outside 1, 2, 4, -2, -4 (the shapes the backward kernel has) are
+    refused on the host, in the constructor and in tune()."""
+    from cctpy_amd.adjoint import AdjointTracker
+    with pytest.raises(ValueError, match="not one of"):
+        AdjointTracker([], ppl=8)
+    tr = AdjointTracker.__new__(AdjointTracker)
+    with pytest.raises(ValueError, match="not among"):
+        tr.tune([1], None, None, None, None, candidates=(1, 3))
