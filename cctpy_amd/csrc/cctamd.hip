@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include <algorithm>
 
@@ -249,15 +250,183 @@ __device__ __forceinline__ void group_tile_terms_f64(const double* __restrict__ 
 constexpr double kNearTile = 6.25;
 constexpr double kFp64Tile = 1.5;
 
+// Occupancy (waves per SIMD) of the scalar-load Biot-Savart kernels without the
+// fused stage, which their fused variants (S >= 0) are held to: the stage
+// update at the end alone would otherwise set their register count (~120-160
+// VGPRs against 44 for k_bs_partial_f32k<1, 8>).  1 = no hint.
+constexpr int kKindF32 = 0, kKindF32k = 1, kKindF64 = 2;
+constexpr int fused_waves(int kind, int ppl, int s) {
+    if (s < 0) return 1;
+    if (kind == kKindF32) return ppl == 1 ? 6 : 4;
+    if (kind == kKindF32k) return ppl == 1 ? 8 : ppl == 2 ? 6 : 4;
+    return ppl == 2 ? 5 : 8;
+}
+
+// ------------------------------------------------------- RK4 stage update
+// Everything the stage update of a particle reads and writes.  scratch rows:
+// 0-2 stage position (read by the Biot-Savart kernels), 3-5 stage velocity,
+// 6-8 running sum of position slopes, 9-11 of velocity slopes, 12-13 the
+// particle's partial-sum slot (so that the update loads it at once), 14 the
+// arrival counters of the fused epilogue (int32, indexed by a point block's
+// first point; zeroed by k_rk4_init, reset by the last arrival).
+struct StageArgs {
+    const double* partial;
+    const double* qs;
+    const int64_t* design_qs_off;
+    const double* bconst;
+    double* state;
+    double* scratch;
+    int64_t ptot;
+    const double* kq;
+    const double* dt;
+    const int64_t* nsteps;
+    int64_t step;
+    double* record;
+    int64_t record_every;
+    double* tape;
+};
+
+__device__ inline int* arrival_counters(const StageArgs& sa) { return (int*)(sa.scratch + 14 * sa.ptot); }
+
+#pragma clang fp contract(off)
+// RK4 stage S of final_code/packages/base_utils.py:303-309 with the callback
+// of particles.py:526-535 for particle g, by one whole wave (its lanes stride
+// the chunk partials; lane 0 writes): a = (e/m) * (v x B), k = [v, a];
+// Y1 = Y0 + dt/2 k1, Y2 = Y0 + dt/2 k2, Y3 = Y0 + dt k3,
+// Y0 += dt/6 (((k1 + 2 k2) + 2 k3) + k4).
+// LEAN (the fused epilogue): the chunk sum first, with two loads in flight per
+// lane, and the particle's state loaded by lane 0 only after it, so that the
+// stage update needs few registers beside the Biot-Savart loop's.  Otherwise
+// every load is issued before the chunk sum (latency first, registers are
+// plentiful in k_rk4_stage).  Both forms compute the same values.
+template <int S, bool LEAN = false>
+__device__ inline void rk4_stage_point(const StageArgs& sa, int64_t g) {
+    const int64_t P = sa.ptot;
+    double* ps = sa.scratch;
+    double* vs = sa.scratch + 3 * P;
+    double* kp = sa.scratch + 6 * P;
+    double* kv = sa.scratch + 9 * P;
+    double p[3], v[3], p0[3], v0[3], sp[3], sv[3], b[3];
+    const PointSlot sl = load_slot(sa.scratch, P, g, sa.partial);
+    int64_t ns;
+    double k, h;
+    AnalyticSet an;
+    const auto load_state = [&]() {
+        an = analytic_set(sl.design, sa.design_qs_off, sa.bconst);
+        ns = sa.nsteps[g];
+        k = sa.kq[g];
+        h = sa.dt[g];
+        for (int c = 0; c < 3; ++c) {
+            const int64_t o = c * P + g;
+            p[c] = ps[o];
+            v[c] = vs[o];
+            p0[c] = sa.state[o];
+            v0[c] = sa.state[3 * P + o];
+            if (S != 0) {
+                sp[c] = kp[o];
+                sv[c] = kv[o];
+            }
+        }
+    };
+    if (LEAN) {
+        chunk_sum<2>(sl, b, sa.scratch);
+        if ((threadIdx.x & (kWave - 1)) != 0) return;
+        load_state();
+        if (sa.step >= ns) return;
+    } else {
+        // The partial-sum slot first (the chunk loads depend on it), then every other load;
+        // the finished-particle test comes after the chunk sum, so that the compiler does not
+        // wait for nsteps before issuing the rest (finished particles keep their state).
+        load_state();
+        chunk_sum(sl, b, sa.scratch);
+        if (sa.step >= ns) return;
+        if ((threadIdx.x & (kWave - 1)) != 0) return;
+    }
+    analytic_fields(an, sa.qs, p, b);
+    if (sa.tape) {  // stage point, velocity and field for the adjoint
+        double* t = sa.tape + S * 9 * P;
+        for (int c = 0; c < 3; ++c) {
+            t[c * P + g] = p[c];
+            t[(3 + c) * P + g] = v[c];
+            t[(6 + c) * P + g] = b[c];
+        }
+    }
+    const double cr[3] = {v[1] * b[2] - v[2] * b[1], -v[0] * b[2] + v[2] * b[0], v[0] * b[1] - v[1] * b[0]};
+    const double a[3] = {cr[0] * k, cr[1] * k, cr[2] * k};
+    const double h2 = h / 2;
+    const bool rec = sa.record && ((sa.step + 1) % sa.record_every == 0);
+    for (int c = 0; c < 3; ++c) {
+        const int64_t o = c * P + g;
+        if (S == 0) {
+            kp[o] = v[c];
+            kv[o] = a[c];
+            ps[o] = p0[c] + v[c] * h2;
+            vs[o] = v0[c] + a[c] * h2;
+        } else if (S == 1) {
+            kp[o] = sp[c] + v[c] * 2;
+            kv[o] = sv[c] + a[c] * 2;
+            ps[o] = p0[c] + v[c] * h2;
+            vs[o] = v0[c] + a[c] * h2;
+        } else if (S == 2) {
+            kp[o] = sp[c] + v[c] * 2;
+            kv[o] = sv[c] + a[c] * 2;
+            ps[o] = p0[c] + v[c] * h;
+            vs[o] = v0[c] + a[c] * h;
+        } else {
+            const double h6 = h / 6;
+            const double np = p0[c] + (sp[c] + v[c]) * h6;
+            const double nv = v0[c] + (sv[c] + a[c]) * h6;
+            sa.state[o] = np;
+            sa.state[3 * P + o] = nv;
+            ps[o] = np;
+            vs[o] = nv;
+            if (rec) {
+                double* r = sa.record + ((sa.step + 1) / sa.record_every) * 6 * P;
+                r[o] = np;
+                r[3 * P + o] = nv;
+            }
+        }
+    }
+}
+#pragma clang fp contract(fast)
+
+// Fused RK4 stage (S >= 0) at the end of a scalar-load Biot-Savart workgroup,
+// after it has written its partials: the workgroup that completes a point
+// block's set of chunks (the last of its w.nchunk items to arrive at the
+// block's counter) sums the block's chunk partials and updates its particles,
+// exactly as k_rk4_stage<S> would, so that a stage is one launch instead of
+// two.  No workgroup waits for another: the others simply end.  Every wave
+// waits until its partial stores are written to its XCD's L2, then after the
+// barrier one agent-scope release by thread 0 writes them back before the
+// count.  Only the last arrival acquires (its L2 and L1 lines invalidated),
+// so that its loads see every other item's stores while the other workgroups
+// keep their cached segment rows.
+template <int S, int NW>
+__device__ inline void stage_after_last_chunk(const Work& w, const StageArgs& sa) {
+    __shared__ int last;
+    __builtin_amdgcn_s_waitcnt(0);  // every counter, vector memory (the partial stores) included
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        int* ctr = arrival_counters(sa) + w.pt_start;
+        last = atomicAdd(ctr, 1) == w.nchunk - 1;
+        if (last) atomicExch(ctr, 0);  // ready for the next launch (all items of the block have arrived)
+    }
+    __syncthreads();
+    if (!last) return;  // workgroup-uniform
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int i = threadIdx.x >> 6; i < w.pt_count; i += NW) rk4_stage_point<S, true>(sa, (int64_t)w.pt_start + i);
+}
+
 // FP32 terms on tiles re-centred at an FP64 origin; each tile's 32 terms are
 // summed in FP32 and the tile sums are accumulated in FP64; near tiles in FP64.
-template <int PPL>
-__global__ __launch_bounds__(kWG) void k_bs_partial_f32(const float* __restrict__ seg32,
+template <int PPL, int S = -1>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(fused_waves(kKindF32, PPL, S)))) void k_bs_partial_f32(const float* __restrict__ seg32,
                                                          const double* __restrict__ tile_org,
                                                          const double* __restrict__ seg64,
                                                          const Work* __restrict__ work,
                                                          const double* __restrict__ pts, int64_t ptot,
-                                                         double* __restrict__ partial) {
+                                                         double* __restrict__ partial, StageArgs sa) {
     constexpr int PB = kWave * PPL;
     const Work w = work[blockIdx.x];
     const int lane = threadIdx.x & (kWave - 1);
@@ -311,6 +480,7 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32(const float* __restrict_
         }
     }
     write_partial<PPL>(ax, ay, az, wave, lane, partial + partial_base(w, PB), w.nchunk);
+    if constexpr (S >= 0) stage_after_last_chunk<S, kWavesPerWG>(w, sa);
 }
 
 // FP32 tracking variant: kl x (q - m) = kl x q - kl x m, so within a tile the
@@ -345,13 +515,13 @@ __device__ inline void tile_direct_term(const float4& a, const float4& b, const 
     bz += (a.x * ry - a.y * rx) * w3;
 }
 
-template <int PPL, int NW>
-__global__ __launch_bounds__(kWave * NW) void k_bs_partial_f32k(const float* __restrict__ seg32k,
+template <int PPL, int NW, int S = -1>
+__global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(fused_waves(kKindF32k, PPL, S)))) void k_bs_partial_f32k(const float* __restrict__ seg32k,
                                                           const double* __restrict__ tile_org,
                                                           const double* __restrict__ seg64,
                                                           const Work* __restrict__ work,
                                                           const double* __restrict__ pts, int64_t ptot,
-                                                          double* __restrict__ partial) {
+                                                          double* __restrict__ partial, StageArgs sa) {
     constexpr int PB = kWave * PPL;
     const Work w = work[blockIdx.x];
     const int lane = threadIdx.x & (kWave - 1);
@@ -426,6 +596,7 @@ __global__ __launch_bounds__(kWave * NW) void k_bs_partial_f32k(const float* __r
         }
     }
     write_partial<PPL, NW>(ax, ay, az, wave, lane, partial + partial_base(w, PB), w.nchunk);
+    if constexpr (S >= 0) stage_after_last_chunk<S, NW>(w, sa);
 }
 
 // 32-point blocks without LDS: each 16-lane row of a wave is a lane group with
@@ -755,11 +926,11 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __rest
 }
 
 // Plain FP64 terms on global-frame segments.
-template <int PPL, int NW>
-__global__ __launch_bounds__(kWave * NW) void k_bs_partial_f64(const double* __restrict__ seg64,
+template <int PPL, int NW, int S = -1>
+__global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(fused_waves(kKindF64, PPL, S)))) void k_bs_partial_f64(const double* __restrict__ seg64,
                                                          const Work* __restrict__ work,
                                                          const double* __restrict__ pts, int64_t ptot,
-                                                         double* __restrict__ partial) {
+                                                         double* __restrict__ partial, StageArgs sa) {
     constexpr int PB = kWave * PPL;
     const Work w = work[blockIdx.x];
     const int lane = threadIdx.x & (kWave - 1);
@@ -789,6 +960,7 @@ __global__ __launch_bounds__(kWave * NW) void k_bs_partial_f64(const double* __r
         }
     }
     write_partial<PPL, NW>(ax, ay, az, wave, lane, partial + partial_base(w, PB), w.nchunk);
+    if constexpr (S >= 0) stage_after_last_chunk<S, NW>(w, sa);
 }
 
 // FP64 split-wave form of k_bs_partial_f64 for designs with few particles
@@ -916,9 +1088,7 @@ __global__ __launch_bounds__(kWG) void k_field_reduce(int ppl, const double* __r
     B[2 * ptot + g] = b[2];
 }
 
-// scratch rows: 0-2 stage position (read by k_bs_partial), 3-5 stage
-// velocity, 6-8 running sum of position slopes, 9-11 of velocity slopes,
-// 12-13 the particle's partial-sum slot (so that stage kernels load it at once).
+// Stage state of every particle from its state (scratch rows as in StageArgs).
 __global__ void k_rk4_init(const double* __restrict__ state, int64_t ptot, double* __restrict__ scratch,
                            double* __restrict__ record, int ppl, const double* __restrict__ partial,
                            const int64_t* __restrict__ design_wbase, const int32_t* __restrict__ design_nchunk,
@@ -931,117 +1101,51 @@ __global__ void k_rk4_init(const double* __restrict__ state, int64_t ptot, doubl
     }
     store_slot(scratch, ptot, g, point_slot(g, ppl, partial, design_wbase, design_nchunk, design_pt_off, nd),
                partial);
+    scratch[14 * ptot + g] = 0.0;  // arrival counters of the fused stages (two int32 per double)
 }
 
-#pragma clang fp contract(off)
-// RK4 stage S of final_code/packages/base_utils.py:303-309 with the callback
-// of particles.py:526-535: a = (e/m) * (v x B), k = [v, a]; Y1 = Y0 + dt/2 k1,
-// Y2 = Y0 + dt/2 k2, Y3 = Y0 + dt k3, Y0 += dt/6 (((k1 + 2 k2) + 2 k3) + k4).
+// RK4 stage S of every particle, one wave per particle (the stage update itself
+// is rk4_stage_point, shared with the fused Biot-Savart epilogue).
 template <int S>
-__global__ __launch_bounds__(kWG) void k_rk4_stage(int ppl, const double* __restrict__ partial,
-                                                   const int64_t* __restrict__ design_wbase,
-                                                   const int32_t* __restrict__ design_nchunk,
-                                                   const int64_t* __restrict__ design_pt_off, int64_t nd,
-                                                   const double* __restrict__ qs,
-                                                   const int64_t* __restrict__ design_qs_off,
-                                                   const double* __restrict__ bconst, double* __restrict__ state,
-                                                   double* __restrict__ scratch, int64_t ptot,
-                                                   const double* __restrict__ kq, const double* __restrict__ dt,
-                                                   const int64_t* __restrict__ nsteps, int64_t step,
-                                                   double* __restrict__ record, int64_t record_every,
-                                                   double* __restrict__ tape) {
+__global__ __launch_bounds__(kWG) void k_rk4_stage(StageArgs sa) {
     const int64_t g = (int64_t)blockIdx.x * kWavesPerWG + (threadIdx.x >> 6);
-    if (g >= ptot) return;
-    double* ps = scratch;
-    double* vs = scratch + 3 * ptot;
-    double* kp = scratch + 6 * ptot;
-    double* kv = scratch + 9 * ptot;
-    // The partial-sum slot first (the chunk loads depend on it), then every other load; the
-    // finished-particle test comes after the chunk sum, so that the compiler does not wait for
-    // nsteps before issuing the rest (finished particles keep their state).
-    const PointSlot sl = load_slot(scratch, ptot, g, partial);
-    const AnalyticSet an = analytic_set(sl.design, design_qs_off, bconst);
-    const int64_t ns = nsteps[g];
-    const double k = kq[g];
-    const double h = dt[g];
-    double p[3], v[3], p0[3], v0[3], sp[3], sv[3];
-    for (int c = 0; c < 3; ++c) {
-        const int64_t o = c * ptot + g;
-        p[c] = ps[o];
-        v[c] = vs[o];
-        p0[c] = state[o];
-        v0[c] = state[3 * ptot + o];
-        if (S != 0) {
-            sp[c] = kp[o];
-            sv[c] = kv[o];
-        }
-    }
-    double b[3];
-    chunk_sum(sl, b, scratch);
-    if (step >= ns) return;
-    if ((threadIdx.x & (kWave - 1)) != 0) return;
-    analytic_fields(an, qs, p, b);
-    if (tape) {  // stage point, velocity and field for the adjoint
-        double* t = tape + S * 9 * ptot;
-        for (int c = 0; c < 3; ++c) {
-            t[c * ptot + g] = p[c];
-            t[(3 + c) * ptot + g] = v[c];
-            t[(6 + c) * ptot + g] = b[c];
-        }
-    }
-    const double cr[3] = {v[1] * b[2] - v[2] * b[1], -v[0] * b[2] + v[2] * b[0], v[0] * b[1] - v[1] * b[0]};
-    const double a[3] = {cr[0] * k, cr[1] * k, cr[2] * k};
-    const double h2 = h / 2;
-    const bool rec = record && ((step + 1) % record_every == 0);
-    for (int c = 0; c < 3; ++c) {
-        const int64_t o = c * ptot + g;
-        if (S == 0) {
-            kp[o] = v[c];
-            kv[o] = a[c];
-            ps[o] = p0[c] + v[c] * h2;
-            vs[o] = v0[c] + a[c] * h2;
-        } else if (S == 1) {
-            kp[o] = sp[c] + v[c] * 2;
-            kv[o] = sv[c] + a[c] * 2;
-            ps[o] = p0[c] + v[c] * h2;
-            vs[o] = v0[c] + a[c] * h2;
-        } else if (S == 2) {
-            kp[o] = sp[c] + v[c] * 2;
-            kv[o] = sv[c] + a[c] * 2;
-            ps[o] = p0[c] + v[c] * h;
-            vs[o] = v0[c] + a[c] * h;
-        } else {
-            const double h6 = h / 6;
-            const double np = p0[c] + (sp[c] + v[c]) * h6;
-            const double nv = v0[c] + (sv[c] + a[c]) * h6;
-            state[o] = np;
-            state[3 * ptot + o] = nv;
-            ps[o] = np;
-            vs[o] = nv;
-            if (rec) {
-                double* r = record + ((step + 1) / record_every) * 6 * ptot;
-                r[o] = np;
-                r[3 * ptot + o] = nv;
-            }
-        }
-    }
+    if (g >= sa.ptot) return;
+    rk4_stage_point<S>(sa, g);
 }
-#pragma clang fp contract(fast)
 
-template <int PPL>
+template <int PPL, int S = -1>
 hipError_t launch_bs(int precision, const void* seg, const double* tile_org, const double* seg_near, const Work* work,
-                     int64_t nwork, const double* pts, int64_t ptot, double* partial, hipStream_t st) {
+                     int64_t nwork, const double* pts, int64_t ptot, double* partial, hipStream_t st,
+                     const StageArgs& sa = StageArgs{}) {
     if (nwork == 0) return hipSuccess;
     if (precision == 32)
-        hipLaunchKernelGGL(k_bs_partial_f32<PPL>, dim3((unsigned)nwork), dim3(kWG), 0, st,
-                           (const float*)seg, tile_org, seg_near, work, pts, ptot, partial);
+        hipLaunchKernelGGL((k_bs_partial_f32<PPL, S>), dim3((unsigned)nwork), dim3(kWG), 0, st,
+                           (const float*)seg, tile_org, seg_near, work, pts, ptot, partial, sa);
     else if (precision == 33)
-        hipLaunchKernelGGL((k_bs_partial_f32k<PPL, kBsWaves>), dim3((unsigned)nwork), dim3(kWave * kBsWaves), 0, st,
-                           (const float*)seg, tile_org, seg_near, work, pts, ptot, partial);
+        hipLaunchKernelGGL((k_bs_partial_f32k<PPL, kBsWaves, S>), dim3((unsigned)nwork), dim3(kWave * kBsWaves), 0,
+                           st, (const float*)seg, tile_org, seg_near, work, pts, ptot, partial, sa);
     else
-        hipLaunchKernelGGL((k_bs_partial_f64<PPL, kBsWaves>), dim3((unsigned)nwork), dim3(kWave * kBsWaves), 0, st,
-                           (const double*)seg, work, pts, ptot, partial);
+        hipLaunchKernelGGL((k_bs_partial_f64<PPL, kBsWaves, S>), dim3((unsigned)nwork), dim3(kWave * kBsWaves), 0,
+                           st, (const double*)seg, work, pts, ptot, partial, sa);
     return hipGetLastError();
+}
+
+// One Biot-Savart pass at the stage points with RK4 stage S fused into the
+// kernel's end (scalar-load kernels: points-per-lane codes 1, 2, 4).
+template <int S>
+hipError_t launch_bs_stage(const DeviceSet& ds, int64_t ptot, double* partial, const StageArgs& sa, hipStream_t st) {
+    const double* pts = sa.scratch;  // rows 0-2: the stage points
+    switch (ds.ppl) {
+        case 1:
+            return launch_bs<1, S>(ds.precision, ds.seg, ds.tile_org, ds.seg_near, ds.work, ds.nwork, pts, ptot,
+                                   partial, st, sa);
+        case 2:
+            return launch_bs<2, S>(ds.precision, ds.seg, ds.tile_org, ds.seg_near, ds.work, ds.nwork, pts, ptot,
+                                   partial, st, sa);
+        default:
+            return launch_bs<4, S>(ds.precision, ds.seg, ds.tile_org, ds.seg_near, ds.work, ds.nwork, pts, ptot,
+                                   partial, st, sa);
+    }
 }
 
 }  // namespace
@@ -1108,6 +1212,20 @@ int field_checked(const DeviceSet& ds, const double* points, int64_t npoints, do
     return hip_status(hipGetLastError());
 }
 
+// Each RK4 stage is a Biot-Savart launch plus k_rk4_stage.  With
+// CCTAMD_FUSED_STAGES=1 in the environment the scalar-load kernels
+// (points-per-lane codes 1, 2, 4) run the stage at their end instead (one
+// launch per stage, the same bits).  Not the default: making every work item's
+// partials visible across the XCDs costs one agent-scope L2 write-back per
+// workgroup (~0.6 us each, serialised within an XCD), far more than the
+// stage launch it saves (~5 us): 2.9x slower on the 368,640-segment AG-CCT
+// section (2,048 items per launch), 6x on the 23,040-segment single CCT.
+bool fused_stages(const DeviceSet& ds) {
+    if (ds.ppl <= 0 || ds.nwork == 0) return false;
+    const char* env = getenv("CCTAMD_FUSED_STAGES");
+    return env && env[0] == '1';
+}
+
 hipError_t launch_rk4_steps(const DeviceSet& ds, double* state, int64_t ptot, const double* kq, const double* dt,
                             const int64_t* nsteps, int64_t step0, int64_t step1, double* scratch, double* partial,
                             double* record, int64_t record_every, double* tape, hipStream_t st) {
@@ -1118,16 +1236,22 @@ hipError_t launch_rk4_steps(const DeviceSet& ds, double* state, int64_t ptot, co
                        ds.design_pt_off, ds.ndesign);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    const bool fused = fused_stages(ds);
+    StageArgs sa{partial, ds.qs, ds.design_qs_off, ds.bconst, state, scratch, ptot, kq, dt, nsteps, 0, record,
+                 record_every, nullptr};
     for (int64_t step = step0; step < step1; ++step) {
-        double* tp = tape ? tape + (step - step0) * 36 * ptot : nullptr;
-#define CCTAMD_STAGE(S)                                                                                          \
-    e = launch_bs_any(ds.precision, ds.ppl, ds.seg, ds.tile_org, ds.seg_near, ds.work, ds.nwork, scratch, ptot, \
-                      partial, st);                                                                              \
-    if (e != hipSuccess) return e;                                                                               \
-    hipLaunchKernelGGL(k_rk4_stage<S>, dim3((unsigned)rblocks), dim3(kWG), 0, st, ds.ppl, partial,               \
-                       ds.design_wbase, ds.design_nchunk, ds.design_pt_off, ds.ndesign, ds.qs, ds.design_qs_off,  \
-                       ds.bconst, state, scratch, ptot, kq, dt, nsteps, step, record, record_every, tp);          \
-    e = hipGetLastError();                                                                                       \
+        sa.step = step;
+        sa.tape = tape ? tape + (step - step0) * 36 * ptot : nullptr;
+#define CCTAMD_STAGE(S)                                                                                           \
+    if (fused) {                                                                                                  \
+        e = launch_bs_stage<S>(ds, ptot, partial, sa, st);                                                        \
+    } else {                                                                                                      \
+        e = launch_bs_any(ds.precision, ds.ppl, ds.seg, ds.tile_org, ds.seg_near, ds.work, ds.nwork, scratch,    \
+                          ptot, partial, st);                                                                     \
+        if (e != hipSuccess) return e;                                                                            \
+        hipLaunchKernelGGL(k_rk4_stage<S>, dim3((unsigned)rblocks), dim3(kWG), 0, st, sa);                        \
+        e = hipGetLastError();                                                                                    \
+    }                                                                                                             \
     if (e != hipSuccess) return e;
         CCTAMD_STAGE(0)
         CCTAMD_STAGE(1)
@@ -1216,7 +1340,10 @@ int64_t cctamd_plan(int64_t ndesign, const int64_t* design_ntile, const int64_t*
     for (int64_t d = 0; d < ndesign; ++d) {
         const int64_t nb = ceil_div(design_npt[d], pb);
         const int64_t T = design_ntile[d];
-        int64_t nc = (nb > 0 && T > 0) ? ceil_div(T, per_item) : 0;
+        // At least one item per point block, also for a design without tiles (an item of no
+        // tiles writes zero partials): the fused RK4 stages update a block's particles at the
+        // end of its items.
+        int64_t nc = nb > 0 ? std::max<int64_t>(1, ceil_div(T, per_item)) : 0;
         // Workgroups go to the XCDs round-robin and item (b, c) is launched as
         // n + b nc + c: with nc a multiple of the XCD count the items of one tile
         // chunk (same segments, other point blocks) share an XCD and its L2.

@@ -267,8 +267,9 @@ __device__ __forceinline__ void chunk_batch(const double* __restrict__ rows, int
 
 // readable: any valid device address, read (not used) in place of the rows of a
 // design without segments (no chunks), so that the first loads need no branch.
+// U chunk loads in flight per lane; the additions are in chunk order whatever U.
+template <int U = 4>
 __device__ inline void chunk_sum(const PointSlot& s, double b[3], const double* __restrict__ readable) {
-    constexpr int U = 4;
     const int lane = threadIdx.x & (kWave - 1);
     const double* rows = s.nc > 0 ? s.rows : readable;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;

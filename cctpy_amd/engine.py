@@ -303,7 +303,7 @@ class DeviceModel:
         kq_t = torch.from_numpy(numpy.ascontiguousarray(kq, numpy.float64)).to(dev)
         dt_t = torch.from_numpy(numpy.ascontiguousarray(dt, numpy.float64)).to(dev)
         ns_t = torch.from_numpy(numpy.ascontiguousarray(ns)).to(dev)
-        scratch = torch.empty((14, plan.npt), dtype=torch.float64, device=dev)
+        scratch = torch.empty((15, plan.npt), dtype=torch.float64, device=dev)
         partial = torch.empty(max(plan.nwork * 3 * points_per_block(ppl), 1), dtype=torch.float64, device=dev)
         rec = torch.empty((max_steps + 1, 6, plan.npt), dtype=torch.float64, device=dev) if record else None
         seg, org, near = self._seg(precision)
@@ -317,9 +317,9 @@ class DeviceModel:
 
     # ------------------------------------------- device-resident variants
     def workspace(self, plan: "Plan") -> Tuple[torch.Tensor, torch.Tensor]:
-        """(scratch [14][P], partial) device buffers for a plan."""
+        """(scratch [15][P], partial) device buffers for a plan."""
         dev = device()
-        return (torch.empty((14, max(plan.npt, 1)), dtype=torch.float64, device=dev),
+        return (torch.empty((15, max(plan.npt, 1)), dtype=torch.float64, device=dev),
                 torch.empty(max(plan.nwork * 3 * points_per_block(plan.ppl), 1), dtype=torch.float64, device=dev))
 
     def track_tensors(self, plan: "Plan", precision: int, state: torch.Tensor, kq: torch.Tensor, dt: torch.Tensor,

@@ -91,6 +91,8 @@ int cctamd_pack_fp32(const double *seg64, int64_t nseg, float *seg32, double *ti
  *   ppl: points per lane (1, 2 or 4): a point block holds 64*ppl points; or -2 / -4:
  *        blocks of 32 / 16 points, each wave split into lane groups walking
  *        different tiles (precisions 33 and 64; not 32).
+ * Every point block gets at least one item (one of no tiles for a design
+ * without CCT tiles).
  * With work == NULL returns the number of work items only. Otherwise fills
  * work[nwork][8] = {design, pt_start, pt_count, tile_start, tile_count, chunk, nchunk, block},
  * design_wbase[d] (first item of design d) and design_nchunk[d].
@@ -156,7 +158,10 @@ int cctamd_bs_partial(int precision, int ppl, const void *seg, const double *til
  * (final_code/packages/gpu_accelerator.py:1912-2044) and the CPU ParticleRunner.run_only
  * (final_code/packages/particles.py:561-632 with base_utils.py:263-311).
  *   state    double[6][P] in/out: x, y, z, vx, vy, vz rows.
- *   scratch  double[14][P] device workspace.
+ *   scratch  double[15][P] device workspace.
+ *   work     a plan of cctamd_plan (every point block has at least one item: with
+ *            points-per-lane codes 1, 2, 4 each RK4 stage runs at the end of the
+ *            Biot-Savart launch, by the last item of each point block to finish).
  *   kq[P] = e/m, dt[P], nsteps[P] (int64) per particle; steps beyond nsteps[i]
  *   leave particle i unchanged; max_steps = max(nsteps).
  *   record   nullable double[max_steps/record_every+1][6][P]: the state at step 0 and
@@ -217,7 +222,7 @@ int cctamd_track_rk4_taped(int precision, int ppl, const void *seg, const double
  *   tgrad: double[nblock][ntile][5] (+=): per tile sum B-bar.dB (current; divide by
  *     I) and dJ/dc_0..3 (tilt coefficients);
  *   qgrad: double[2*Q][P] (+=): dJ/d(gradient), dJ/d(second gradient) of each QS;
- *   workspaces: state_ws[6][P], scratch[14][P], partial (as cctamd_bs_field),
+ *   workspaces: state_ws[6][P], scratch[15][P], partial (as cctamd_bs_field),
  *     tape[ckpt_every][36][P] (or the full tape), adj_ws[15][P], bs_pts[3][P], bs_wts[3][P].
  */
 int cctamd_track_rk4_adjoint(int precision, int ppl, const void *seg, const double *tile_org,

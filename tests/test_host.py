@@ -46,16 +46,26 @@ def _plan(ntile, npt, ppl, target, min_tiles=4):
 
 @pytest.mark.parametrize("ppl", [1, 2, 4, -2, -4])
 def test_plan_covers_every_point_tile_pair_once(ppl):
+    """Every (tile, point) pair of a design lies in exactly one work item, and every point
+    block has at least one item (a design without tiles gets one item of no tiles per block:
+    the fused RK4 stages update a block's particles at the end of its last item)."""
     ntile = [37, 0, 5, 1200, 3]
     npt = [300, 17, 0, 64, 1]
     work, wbase, nch, toff, poff = _plan(ntile, npt, ppl, target=200)
     pb = 64 * ppl if ppl > 0 else 64 // -ppl
     cover = {}
+    blocks = set()
     for d in range(len(ntile)):
         nb = -(-npt[d] // pb)
-        if ntile[d] == 0 or npt[d] == 0:
+        if npt[d] == 0:
             assert nch[d] == 0
             continue
+        if ntile[d] == 0:
+            assert nch[d] == 1
+        for b in range(nb):
+            for c in range(nch[d]):
+                w = work[wbase[d] + b * nch[d] + c]
+                blocks.add(int(w[1]))
         for b in range(nb):
             for c in range(nch[d]):
                 w = work[wbase[d] + b * nch[d] + c]
@@ -66,6 +76,7 @@ def test_plan_covers_every_point_tile_pair_once(ppl):
                         cover[(t, q)] = cover.get((t, q), 0) + 1
     expect = {(toff[d] + t, poff[d] + q) for d in range(len(ntile)) for t in range(ntile[d]) for q in range(npt[d])}
     assert set(cover) == expect and all(v == 1 for v in cover.values())
+    assert blocks == {poff[d] + b * pb for d in range(len(ntile)) for b in range(-(-npt[d] // pb))}
 
 
 
