@@ -259,7 +259,7 @@ constexpr int fused_waves(int kind, int ppl, int s) {
     if (s < 0) return 1;
     if (kind == kKindF32) return ppl == 1 ? 6 : 4;
     if (kind == kKindF32k) return ppl == 1 ? 8 : ppl == 2 ? 6 : 4;
-    return ppl == 2 ? 5 : 8;
+    return ppl == 1 ? 8 : ppl == 2 ? 5 : 4;
 }
 
 // ------------------------------------------------------- RK4 stage update
@@ -502,6 +502,41 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(fused_waves
 #endif
 constexpr int kBsWaves = CCTAMD_BS_WAVES;
 
+// One segment's K/C row as the K/C loop uses it (the row's last two floats are
+// padding): a = kl.xyz, (kl x m).x; b = (kl x m).yz, -2m.xy; c = -2m.z, |m|^2.
+struct KCSeg {
+    float4 a, b;
+    float2 c;
+};
+
+__device__ __forceinline__ KCSeg kc_seg(const float4* __restrict__ s, int k) {
+    KCSeg g;
+    g.a = s[3 * k];
+    g.b = s[3 * k + 1];
+    g.c = *reinterpret_cast<const float2*>(s + 3 * k + 2);
+    return g;
+}
+
+// The K/C pair terms of one segment for a lane's PPL points (13 VALU slots each).
+template <int PPL>
+__device__ __forceinline__ void kc_terms(const KCSeg& g, const float (&qx)[PPL], const float (&qy)[PPL],
+                                         const float (&qz)[PPL], const float (&qq)[PPL], float (&kx)[PPL],
+                                         float (&ky)[PPL], float (&kz)[PPL], float (&cx)[PPL], float (&cy)[PPL],
+                                         float (&cz)[PPL]) {
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+        const float r2 = __builtin_fmaf(qz[j], g.c.x, __builtin_fmaf(qy[j], g.b.w, __builtin_fmaf(qx[j], g.b.z, qq[j] + g.c.y)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float w3 = inv * inv * inv;
+        kx[j] += g.a.x * w3;
+        ky[j] += g.a.y * w3;
+        kz[j] += g.a.z * w3;
+        cx[j] += g.a.w * w3;
+        cy[j] += g.b.x * w3;
+        cz[j] += g.b.y * w3;
+    }
+}
+
 // One FP32 direct-form term (kl x r) |r|^-3 from a K/C row, r = q - m.
 __device__ inline void tile_direct_term(const float4& a, const float4& b, const float4& c, float qx, float qy,
                                         float qz, float& bx, float& by, float& bz) {
@@ -571,23 +606,7 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(fuse
             continue;
         }
 #pragma unroll 8
-        for (int k = 0; k < kTile; ++k) {
-            const float4 a = s[3 * k];      // kl.xyz, (kl x m).x
-            const float4 b = s[3 * k + 1];  // (kl x m).yz, -2m.xy
-            const float4 c = s[3 * k + 2];  // -2m.z, |m|^2
-#pragma unroll
-            for (int j = 0; j < PPL; ++j) {
-                const float r2 = __builtin_fmaf(qz[j], c.x, __builtin_fmaf(qy[j], b.w, __builtin_fmaf(qx[j], b.z, qq[j] + c.y)));
-                const float inv = __builtin_amdgcn_rsqf(r2);
-                const float w3 = inv * inv * inv;
-                kx[j] += a.x * w3;
-                ky[j] += a.y * w3;
-                kz[j] += a.z * w3;
-                cx[j] += a.w * w3;
-                cy[j] += b.x * w3;
-                cz[j] += b.y * w3;
-            }
-        }
+        for (int k = 0; k < kTile; ++k) kc_terms<PPL>(kc_seg(s, k), qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             ax[j] += (double)((ky[j] * qz[j] - kz[j] * qy[j]) - cx[j]);
