@@ -92,6 +92,18 @@ __device__ inline float wave_sum(float v) {
 // tgrad[block][tile] = {sum b . dB (current), dJ/dc_0..3 (tilt coefficients)}.
 // FP32 terms on re-centred tiles, FP64 across tiles.
 
+// Segments per iteration of the backward kernels' tile loops, as measured
+// fastest: 64-point blocks 4 with one point per lane and 2 with more (the
+// adjoint pass of a 256-proton AG-CCT section: 17 % faster than 2 with one
+// point per lane, 3 % slower with two), 16-point blocks 2, 32-point blocks 1
+// (5 % faster than 2 on the 64-design population).
+#ifndef CCTAMD_BWD1_UNROLL
+#define CCTAMD_BWD1_UNROLL 4
+#endif
+constexpr int bwd_unroll(int ppl) { return ppl == 1 ? CCTAMD_BWD1_UNROLL : 2; }
+constexpr int kBwdGrpUnroll = 2;
+constexpr int kBwdGrp2Unroll = 1;
+
 // One point x segment term. hb = b / 2; returns c = r x b, w = |r|^-3,
 // h = u / 2 and e = w kl . c.
 struct BwdTerm {
@@ -160,6 +172,7 @@ __global__ __launch_bounds__(kWG) void k_bs_backward_f32(
     const double* __restrict__ pts, const double* __restrict__ wts, int64_t ptot, int64_t ntile_total,
     double* __restrict__ partial, double* __restrict__ tgrad) {
     constexpr int PB = kWave * PPL;
+    constexpr int kUnroll = bwd_unroll(PPL);
     const Work w = work[blockIdx.x];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -201,7 +214,7 @@ __global__ __launch_bounds__(kWG) void k_bs_backward_f32(
         TanAcc ta{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         const float4* s = reinterpret_cast<const float4*>(seg32) + (int64_t)t * kTile * 2;
         const float4* tn = reinterpret_cast<const float4*>(seg_tan) + (int64_t)t * kTanRows * 2;
-#pragma unroll 2
+#pragma unroll kUnroll
         for (int k = 0; k < kTile; ++k) {
             const float4 kl = s[2 * k];
             const float4 m = s[2 * k + 1];
@@ -306,7 +319,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(G == 2 ? 5 
             TanAcc ta{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             const float4* sr = lseg[stream];
             const float4* tr = ltan[stream];
-#pragma unroll 2
+#pragma unroll kBwdGrpUnroll
             for (int k = 0; k < kTile; ++k) {
                 const BwdTerm b = bwd_term(qx, qy, qz, bx, by, bz, hbx, hby, hbz, sr[2 * k], sr[2 * k + 1]);
                 sx += b.hx;
@@ -425,7 +438,7 @@ __global__ __launch_bounds__(kWG) void k_bs_backward_f32_grp2(
             TanAcc ta{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             const float4* sr = lseg[stream];
             const float4* tr = ltan[stream];
-#pragma unroll 2
+#pragma unroll kBwdGrp2Unroll
             for (int k = 0; k < kTile; ++k) {
                 const float4 kl = sr[2 * k], m = sr[2 * k + 1];
                 float gkx, gky, gkz, ghx, ghy, ghz;
