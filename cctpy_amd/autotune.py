@@ -49,7 +49,7 @@ def _store(key: str, val: dict) -> None:
 
 # Bumped whenever the kernels or the candidate set change, so that cached choices of an
 # older library are not reused.
-KERNEL_SET = 8
+KERNEL_SET = 9
 TIE = 0.015
 
 

@@ -502,6 +502,23 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(fused_waves
 #endif
 constexpr int kBsWaves = CCTAMD_BS_WAVES;
 
+// Segments per iteration of the K/C, 16-point K/C, FP64 and split-wave FP64
+// tile loops.  Measured on the AG-CCT section: 4 is 1.5 % faster than 8 for the
+// K/C kernel (16: 1.5 % slower with one point per lane, 9 % with two); 2 or 8
+// for the others stayed within 1 % of 4.
+#ifndef CCTAMD_KC_UNROLL
+#define CCTAMD_KC_UNROLL 4
+#endif
+#ifndef CCTAMD_KCG_UNROLL
+#define CCTAMD_KCG_UNROLL 4
+#endif
+#ifndef CCTAMD_F64_UNROLL
+#define CCTAMD_F64_UNROLL 4
+#endif
+#ifndef CCTAMD_F64G_UNROLL
+#define CCTAMD_F64G_UNROLL 4
+#endif
+
 // One segment's K/C row as the K/C loop uses it (the row's last two floats are
 // padding): a = kl.xyz, (kl x m).x; b = (kl x m).yz, -2m.xy; c = -2m.z, |m|^2.
 struct KCSeg {
@@ -605,7 +622,7 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(fuse
             }
             continue;
         }
-#pragma unroll 8
+#pragma unroll CCTAMD_KC_UNROLL
         for (int k = 0; k < kTile; ++k) kc_terms<PPL>(kc_seg(s, k), qx, qy, qz, qq, kx, ky, kz, cx, cy, cz);
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
@@ -893,7 +910,7 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f32k_grp(const float* __rest
             __syncthreads();
             continue;
         }
-#pragma unroll 4
+#pragma unroll CCTAMD_KCG_UNROLL
         for (int k = 0; k < kTile; ++k) {
             const float4 a = r4[3 * k];
             const float4 b = r4[3 * k + 1];
@@ -963,7 +980,7 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(fuse
     const int tend = w.tile_start + w.tile_count;
     for (int t = w.tile_start + wave; t < tend; t += NW) {
         const double2* s = reinterpret_cast<const double2*>(seg64) + (int64_t)t * kTile * 4;
-#pragma unroll 4
+#pragma unroll CCTAMD_F64_UNROLL
         for (int k = 0; k < kTile; ++k) {
             const double2 k01 = s[4 * k], k2_ = s[4 * k + 1], m01 = s[4 * k + 2], m2_ = s[4 * k + 3];
 #pragma unroll
@@ -1041,7 +1058,7 @@ __global__ __launch_bounds__(kWG) void k_bs_partial_f64_grp(const double* __rest
             __syncthreads();
             if (t0 + stream < tend) {  // uniform within a lane group
                 const double2* r = reinterpret_cast<const double2*>(lds[stream]);
-#pragma unroll 4
+#pragma unroll CCTAMD_F64G_UNROLL
                 for (int k = 0; k < SEGS; ++k) {
                     const double2 k01 = r[3 * k], k2m0 = r[3 * k + 1], m12 = r[3 * k + 2];
 #pragma unroll
