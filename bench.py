@@ -116,6 +116,7 @@ def cpu_baseline(workload: str, sample: str, workers: int = 16):
     rate = r["numpy_pool"]["particle_steps_per_s"]
     pairs = 4 * r["segments"]
     return {"value": rate, "unit": "particle-steps/s", "cores": workers, "kind": "port",
+            "os_cpu_count": os.cpu_count(), "cores_visible": cores, "cpu_model": model,
             "sample": f"{np_} protons x {sp} RK4 steps of {workload} over all {r['segments']} segments + QS: numpy "
                       f"restatement of the reference CPU path (per-particle RK4, numpy Biot-Savart per CCT), "
                       f"{workers} processes, one task per proton ({r['numpy_pool']['seconds']:.1f} s); host cores "
@@ -414,13 +415,17 @@ def main() -> int:
     if args.copy_mib:
         return copy_calibration(args.copy_mib)
 
-    # the CPU baseline first, in a child process, before this process touches the GPU
-    cpu = None
+    # the CPU baselines first, in child processes, before this process touches the GPU
+    cpu = cpu_cfg1 = None
     if not (args.no_cpu or args.roofline_only or args.mode == "adjoint"):
         cpu = cpu_baseline(args.workload, args.cpu_sample)
         log(f"[bench] cpu baseline: numpy {cpu['value']:.4g} particle-steps/s ({cpu['cores']} processes), "
             f"{cpu['numpy_1_process']['particle_steps_per_s']:.3g} in one process; C/OpenMP "
             f"{cpu['c_openmp']['particle_steps_per_s']:.4g}")
+        if not args.no_extras:  # the single CCT's whole run (64 protons x 498 steps), not a slice
+            cpu_cfg1 = cpu_baseline("single_cct", "1x498,64x498")
+            log(f"[bench] cpu single_cct in full: numpy {cpu_cfg1['value']:.4g} particle-steps/s "
+                f"({cpu_cfg1['cores']} processes); C/OpenMP {cpu_cfg1['c_openmp']['particle_steps_per_s']:.4g}")
 
     import numpy as np
     import torch
@@ -502,6 +507,8 @@ def main() -> int:
                 "kernel": kname, "avg_launch_ms": avg_ms, "flop_per_launch": flops,
                 "algorithmic_bytes_per_launch": float(model.nseg_padded * 48 + plan32.npt * 24
                                                       + plan32.nwork * 3 * engine.points_per_block(plan32.ppl) * 8)}
+    if roofline["traffic"] is not None:  # HBM bytes per launch (counters) at this launch duration
+        roofline["traffic_GB_per_s"] = roofline["traffic"] / (avg_ms / 1e3) / 1e9
     log(f"[bench] bs kernel: {avg_ms * 1e3:.1f} us/launch, {achieved:.2f} TFLOP/s "
         f"({100 * roofline['frac']:.1f}% of FP32 vector peak)")
 
@@ -557,6 +564,10 @@ def main() -> int:
         A_P, A_seg, A_steps = A_.P, int(A_.model.nseg), A_.max_steps
         del A_
         extras["single_cct_tracking"] = tracking_leg("single_cct", passes=5)
+        if cpu_cfg1 is not None:
+            extras["single_cct_tracking"]["cpu_full_run"] = cpu_cfg1
+            extras["single_cct_tracking"]["fp32_vs_cpu_numpy"] = \
+                extras["single_cct_tracking"]["fp32"]["particle_steps_per_s"] / cpu_cfg1["value"]
         extras["hust_total"] = tracking_leg("hust_total", max_steps=2000, passes=1, ppl=2, items_per_cu=4,
                                             adjoint_steps=400)
         extras["single_cct_field_along_arc"] = field_along_arc()
