@@ -705,6 +705,8 @@ __device__ __forceinline__ void direct_row_steps(const KCRow& g, const float (&q
     if constexpr (S + 1 < 16) direct_row_steps<S + 1>(g, qx, qy, qz, bx, by, bz);
 }
 
+// 6 waves per SIMD (80 VGPRs): hints of 7 (72 VGPRs, 10 spilled) and 8 (64, 23
+// spilled) measured 8 % and 11 % slower on the 64-design population.
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(6))) void k_bs_partial_f32k_row(const float* __restrict__ seg32k,
                                                              const double* __restrict__ tile_org,
                                                              const double* __restrict__ seg64,
