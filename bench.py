@@ -592,8 +592,8 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "dtype_detail": "f32 Biot-Savart pair terms and 32-segment tile sums; f64 tile origins, sums across tiles "
-                        "and RK4 state",
+        "dtype_detail": "f32 Biot-Savart pair terms and 32-segment tile sums; f64 terms for tiles a point lies "
+                        "among, f64 tile origins, sums across tiles and RK4 state",
         "data": "synthetic",
         "config": {"workload": wl.name, "designs": len(S.bls), "segments": int(model.nseg), "protons": S.P,
                    "rk4_steps": S.max_steps, "steps_capped": args.max_steps > 0,

@@ -272,3 +272,13 @@ def test_adjoint_tracker_refuses_unknown_block_shapes_before_any_device_work():
     tr = AdjointTracker.__new__(AdjointTracker)
     with pytest.raises(ValueError, match="not among"):
         tr.tune([1], None, None, None, None, candidates=(1, 3))
+
+
+@pytest.mark.parametrize("ppl", [0, 3, 8, -1, -8])
+def test_plan_refuses_unknown_block_shapes(ppl):
+    """Points-per-lane codes other than 1, 2, 4, -2, -4 have no kernel: the planner returns
+    CCTAMD_EINVAL = -22 (a host-only call)."""
+    one = np.ones(1, np.int64)
+    zero = np.zeros(1, np.int64)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    assert N.lib.cctamd_plan(1, p(one), p(one), p(zero), p(zero), ppl, 16, 1, None, None, None) == -22
