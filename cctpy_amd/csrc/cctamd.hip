@@ -1253,11 +1253,12 @@ int field_checked(const DeviceSet& ds, const double* points, int64_t npoints, do
 // Each RK4 stage is a Biot-Savart launch plus k_rk4_stage.  With
 // CCTAMD_FUSED_STAGES=1 in the environment the scalar-load kernels
 // (points-per-lane codes 1, 2, 4) run the stage at their end instead (one
-// launch per stage, the same bits).  Not the default: making every work item's
-// partials visible across the XCDs costs one agent-scope L2 write-back per
-// workgroup (~0.6 us each, serialised within an XCD), far more than the
-// stage launch it saves (~5 us): 2.9x slower on the 368,640-segment AG-CCT
-// section (2,048 items per launch), 6x on the 23,040-segment single CCT.
+// launch per stage, the same bits).  Not the default, being slower: the stage
+// update runs at the end of the launch in one workgroup per point block, whose
+// waves sum their points' chunk partials one point after another (the chunk
+// order of k_rk4_stage), a serial tail of ~45 us per launch on the 23,040-segment
+// single CCT whatever the item count (6x slower there; 2.9x on the
+// 368,640-segment AG-CCT section), against ~5 us for the stage kernel.
 bool fused_stages(const DeviceSet& ds) {
     if (ds.ppl <= 0 || ds.nwork == 0) return false;
     const char* env = getenv("CCTAMD_FUSED_STAGES");
