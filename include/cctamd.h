@@ -114,6 +114,9 @@ int64_t cctamd_plan(int64_t ndesign, const int64_t *design_ntile, const int64_t 
  *              tile_org: the same with per-tile sums K = sum w kl, C = sum w kl x m,
  *              B_tile = K x q - C and |q-m|^2 = (|q|^2 + |m|^2) - 2 q.m); at both FP32
  *              precisions tiles near a point are summed in FP64 from seg_near.
+ *   seg, tile_org, seg_near: device pointers, required whenever nwork > 0 (a plan of
+ *            cctamd_plan has an item for every point block, also when the designs have no
+ *            segments: pass buffers of at least one row then; items without tiles read none).
  *   partial: device workspace of nwork*3*PB doubles (PB = points per block); design d's
  *            partials start at design_wbase[d]*3*PB, laid out [block][point][component][chunk].
  */
