@@ -18,7 +18,7 @@ import torch
 from . import engine
 
 PPL = (1, 2, 4)
-ITEMS_PER_CU = (2, 4, 8, 16)
+ITEMS_PER_CU = (2, 4, 8, 16, 32)  # 32: 2-3 % faster on the 64-design population's uneven designs
 _MEM: Dict[str, dict] = {}
 
 
@@ -49,7 +49,7 @@ def _store(key: str, val: dict) -> None:
 
 # Bumped whenever the kernels or the candidate set change, so that cached choices of an
 # older library are not reused.
-KERNEL_SET = 9
+KERNEL_SET = 10
 TIE = 0.015
 
 
